@@ -1,0 +1,186 @@
+/*
+ * scft.h -- C ABI of libscft_hip.so, the MI355X (gfx950) implementation of
+ * giantsda/SCFT's inner field evaluation: linear-FEM assembly, the
+ * implicit-Euler propagator march, the Romberg density phi(x) and the
+ * residual phi - phi_t, plus the outer Anderson/Broyden loops that drive the
+ * residual to zero.  All arithmetic on the solver path is FP64.
+ *
+ * Plain C types only (pointers, sizes, doubles); every function returns an
+ * int status (SCFT_OK = 0) unless it replaces a reference symbol whose
+ * signature says otherwise.  Nothing here calls exit().
+ *
+ * Index conventions
+ *   node i = 0..M-1 of a mesh with M nodes (m = M-1 elements); x_0 = 0 is the
+ *   hard wall.  The unknowns are the nodes that are not Dirichlet:
+ *     SCFT_BC_DD (walls at both ends, the reference problem): nodes 1..M-2
+ *     SCFT_BC_DN (wall at 0, zero-flux end at L):              nodes 1..M-1
+ *   Arrays "on unknowns" hold n values, entry u <-> node u+1.
+ */
+#ifndef SCFT_H
+#define SCFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define SCFT_OK 0
+#define SCFT_E_ARG 1      /* invalid argument (sizes, N not 2^k >= 16, ...) */
+#define SCFT_E_HIP 2      /* HIP runtime error (no device, launch failure)  */
+#define SCFT_E_NAN 3      /* non-finite value produced by an evaluation     */
+#define SCFT_E_PIVOT 4    /* non-positive tridiagonal pivot (D not an M-matrix) */
+#define SCFT_E_NOMEM 5    /* device or host allocation failed               */
+#define SCFT_E_STATE 6    /* call out of order (no init / no context)       */
+#define SCFT_E_NOCONV 7   /* outer solver hit its iteration limit           */
+
+#define SCFT_BC_DD 0
+#define SCFT_BC_DN 1
+#define SCFT_C_ROWSCALED 0  /* C_ij = w_i A_ij (1D_FEM.c:104-105)            */
+#define SCFT_C_CONSISTENT 1 /* C_ij = int w_h phi_i phi_j (scft.cc:641-651)   */
+
+/* per-instance status word bits (scft_batch_* outputs) */
+#define SCFT_ST_CONVERGED 1u
+#define SCFT_ST_NAN 2u
+#define SCFT_ST_PIVOT 4u
+#define SCFT_ST_MAXITER 8u
+#define SCFT_ST_FALLBACK 16u /* warm start rejected, restarted cold */
+
+/* ---- library lifetime ---------------------------------------------------
+ * Replaces PetscInitialize/PetscFinalize around the evaluator
+ * (1D_FEM.c:295,363).  scft_init is optional: the drop-in evaluator
+ * initialises lazily with the defaults below. */
+typedef struct scft_config {
+  int device;        /* HIP device ordinal (default 0)                      */
+  int N;             /* contour steps, 2^k >= 16 (default 2048, drivescft.cc:130) */
+  double L;          /* domain length (default 3.72374357332160)            */
+  double tau;        /* target wall width (default 0.530252230020752)       */
+  int bc;            /* SCFT_BC_DD (default, the reference problem) / _DN   */
+  int cmat;          /* SCFT_C_ROWSCALED (default) / SCFT_C_CONSISTENT      */
+  double beta;       /* mixing used by scft_fixed_point (default 0.1)       */
+  int verbose;       /* print each evaluation like 1D_FEM.c:280-283 (default 0) */
+} scft_config;
+
+void scft_default_config(scft_config *cfg);
+int scft_init(const scft_config *cfg);
+int scft_finalize(void);
+const char *scft_strerror(int code);
+int scft_last_status(void); /* status of the last drop-in evaluation */
+
+/* ---- reference drop-in evaluator ----------------------------------------
+ * Replaces `int simple_FEM_1D_transient(int n, double *w, double *out)`
+ * (1D_FEM.c:47-287): n interior unknowns, NR 1-based w[1..n] in and
+ * out[1..n] = phi - phi_t out (1D_FEM.c:274-277), walls at w = 0
+ * (1D_FEM.c:65-66).  Mode R numerics: N steps paired q_j q_{N-j} and
+ * Romberg-integrated as DEALII_SCFT/drivescft.cc:184-193.  The target is the
+ * caller's global `f0_given` (0-based, n+2 values, 1D_FEM.c:14) when that
+ * pointer is non-NULL, else the position-based profile of
+ * DEALII_SCFT/src/scft.cc:188-215 (or whatever scft_set_target installed).
+ * Increments `local_iteration` (1D_FEM.c:279).  On failure sets the
+ * caller-visible `funcerr` (broydn.c:25,37,82) and returns nonzero.
+ * Synchronous: returns after the device work and the copy back. */
+int simple_FEM_1D_transient(int n, double *w, double *out);
+extern double *f0_given;
+extern int local_iteration;
+extern int funcerr;
+
+/* Install the target used by the drop-in entries when f0_given is NULL:
+ * phi_t at all n+2 nodes (0-based).  NULL restores the built-in profile. */
+int scft_set_target(int n_nodes, const double *phi_t);
+
+/* adm.c callback form (adm.c:25-27, funcvmix): xnew = x + beta (phi[x] - phi_t)
+ * on the n unknowns, x[0..n-1] (flag 0) or x[1..n] (flag 1, NR offset). */
+void scft_fixed_point(int n, double *x, double *xnew);
+void scft_fixed_point_nr(int n, double *x, double *xnew);
+
+/* adm_chen / deal.II callback form (NR_chen.h:31-34, drivescft.cc:218-243):
+ * 0-based, out = phi_t - phi (drivescft.cc:210-215 sign). */
+void scft_adm_chen_f(int n, double *in, double *out);
+
+/* ---- batched context ----------------------------------------------------
+ * Many independent instances of uneven size evaluated in one persistent
+ * launch per bucket of sizes.  Instance k: M[k] nodes at x + xoff[k]
+ * (xoff in doubles into the concatenated node array), N[k] contour steps
+ * (2^k >= 16), target phi_t on its unknowns (concatenated, instance k at
+ * uoff[k] as reported by scft_batch_layout).  All inputs are copied. */
+typedef struct scft_batch scft_batch;
+
+typedef struct scft_batch_desc {
+  int n_inst;
+  const int *M;        /* nodes per instance                                */
+  const int *N;        /* contour steps per instance                        */
+  const long long *xoff;/* offset of instance k's nodes in x                */
+  const double *x;     /* concatenated node coordinates                     */
+  const double *phi_t; /* concatenated targets on unknowns (uoff layout)    */
+  int bc;              /* SCFT_BC_DD / SCFT_BC_DN (all instances)           */
+  int cmat;            /* SCFT_C_ROWSCALED / SCFT_C_CONSISTENT              */
+  const char *tune_file; /* autotuner table to read/write, or NULL          */
+} scft_batch_desc;
+
+int scft_batch_create(scft_batch **out, const scft_batch_desc *desc);
+int scft_batch_destroy(scft_batch *b);
+/* offsets of each instance's unknowns (length n_inst) and the total */
+int scft_batch_layout(const scft_batch *b, long long *uoff, long long *total);
+
+/* One field evaluation of every instance (host buffers on unknowns):
+ * phi (nullable), r = phi - phi_t, rnorm[k] = ||r_k||_inf, status[k]. */
+int scft_batch_eval(scft_batch *b, const double *w, double *phi, double *r,
+                    double *rnorm, unsigned *status);
+
+/* Diagnostic for parity tests: the q-slabs 1..N/2-1 kept in device scratch
+ * by the last evaluation of a single-instance batch, in node order
+ * (out[(s-1)*n + u] = q(x_{u+1}, s/N)); out holds (N/2-1)*n doubles. */
+int scft_debug_slabs(scft_batch *b, int inst, double *out);
+
+/* Device-resident safeguarded Anderson mixing of every instance until
+ * ||r||_inf <= tol or max_iter evaluations:
+ *   w_{k+1} = wbar + beta rbar, theta from the history Gram system,
+ *   r mixed as is / relative / log(phi/phi_t), the step capped in inf-norm,
+ *   history cleared on growth past restart_factor * best or singular U. */
+typedef struct scft_anderson_opts {
+  int history;            /* 10 */
+  double beta;            /* 0.1 */
+  double tol;             /* 1e-6 */
+  int max_iter;           /* evaluation limit per instance */
+  double step_cap;        /* inf-norm cap of w_{k+1} - w_k (<= 0: none) */
+  double restart_factor;  /* growth restart (huge = never) */
+  double regularization;  /* U += reg * trace(U) * I */
+  int transform;          /* 0 plain, 1 relative, 2 log */
+  double floor;           /* phi_t floor used by the transforms */
+  double fallback_factor; /* warm starts whose first ||r|| exceed
+                             fallback_factor * median restart cold (<=0: off) */
+  int check_every;        /* host polls the active count every k sweeps */
+} scft_anderson_opts;
+
+void scft_default_anderson(scft_anderson_opts *o);
+
+typedef struct scft_solve_stats {
+  int sweeps;             /* batched sweeps launched */
+  int converged;          /* instances with ||r|| <= tol */
+  double seconds;         /* wall time of the solve (device resident) */
+  double eval_seconds;    /* part spent in the march kernels (events) */
+  double node_steps;      /* sum over evaluations of n * N */
+  double bytes;           /* sum over evaluations of the algorithmic bytes */
+} scft_solve_stats;
+
+/* w: in = initial fields, out = solutions (host, on unknowns); w_cold
+ * (nullable) = cold-start fields used by the fallback; iters[k] / status[k]
+ * (nullable) per instance. */
+int scft_batch_solve(scft_batch *b, const scft_anderson_opts *o, double *w,
+                     const double *w_cold, int *iters, unsigned *status,
+                     scft_solve_stats *stats);
+
+/* Timing helpers used by bench.py: time `reps` evaluations of the current
+ * device field with HIP events on the launch stream. */
+int scft_batch_time_eval(scft_batch *b, const double *w, int reps,
+                         double *ms_per_eval, double *march_ms);
+
+/* Autotuner: time every (nodes-per-lane, waves) variant that fits each size
+ * bucket and keep the fastest; written to desc.tune_file when given. */
+int scft_batch_autotune(scft_batch *b, int reps);
+int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
+                      int *K, int *W, int max_buckets);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,46 @@
+/*
+ * oracle_batch.h -- CPU drivers around the oracle evaluator.
+ * TEST INFRASTRUCTURE ONLY.
+ */
+#ifndef SCFT_ORACLE_BATCH_H
+#define SCFT_ORACLE_BATCH_H
+
+#include "oracle_solvers.h"
+#include "scft_oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_eval_ctx {
+  orc_problem p;
+  double beta;
+  double *wfull, *phi, *rfull, *q;
+  long nevals;
+} orc_eval_ctx;
+
+int orc_ctx_init(orc_eval_ctx *c, const orc_problem *p, double beta);
+void orc_ctx_free(orc_eval_ctx *c);
+int orc_residual_cb(int n, const double *w, double *r, void *ctx);
+void orc_fixed_point_cb(int n, const double *x, double *xnew, void *ctx);
+int orc_residual_dealii_cb(int n, const double *w, double *r, void *ctx);
+
+int orc_solve_anderson(const orc_problem *p, double *w_full,
+                       const orc_anderson_opts *o, int precond,
+                       double pfloor, orc_anderson_stats *st);
+
+int orc_batch_solve(int n_inst, const int *M, const int *N, const long *xoff,
+                    const double *x, const double *phi_t, double *w, int bc,
+                    int cmat, const orc_anderson_opts *o, int precond,
+                    double pfloor, int threads, int *iters, int *status,
+                    double *final_err);
+
+int orc_batch_eval(int n_inst, const int *M, const int *N, const long *xoff,
+                   const double *x, const double *phi_t, const double *w,
+                   int bc, int cmat, int threads, double *phi, double *r,
+                   double *rnorm);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,70 @@
+/*
+ * oracle_solvers.h -- CPU restatement of the reference's outer solvers
+ * (gaussj, adm, broydn) and of the safeguarded Anderson mixer.
+ * TEST INFRASTRUCTURE ONLY.
+ */
+#ifndef SCFT_ORACLE_SOLVERS_H
+#define SCFT_ORACLE_SOLVERS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_GJ_PERTURB = 0, ORC_GJ_RETURN = 1 };
+
+int orc_gaussj(double *a, int n, double *b, int m, int zero_pivot_mode);
+
+/* fixed-point map x -> xnew (adm.c's funcvmix, 0-based) */
+typedef void (*orc_fixed_point_fn)(int n, const double *x, double *xnew,
+                                   void *ctx);
+int orc_adm(double *x, int n, orc_fixed_point_fn f, void *ctx, double tolf,
+            int maxits, double *xnew, int *its_out, double *err_out);
+
+/* residual map w -> r (nonzero return = evaluation failure) */
+typedef int (*orc_residual_fn)(int n, const double *w, double *r, void *ctx);
+
+typedef struct orc_anderson_opts {
+  int history;          /* previous iterates kept (10) */
+  double beta;          /* mixing (0.1) */
+  double tol;           /* stop at ||r||_inf <= tol */
+  int max_iter;         /* evaluations allowed */
+  double step_cap;      /* ||w_{k+1}-w_k||_inf cap (<= 0: none) */
+  double restart_factor;/* clear history when ||r|| > factor * best */
+  double regularization;/* U += reg * trace(U) * I before the solve */
+  int transform;        /* ORC_MIX_PLAIN / _RELATIVE / _LOG */
+  double floor;         /* phi_t floor used by the transforms */
+} orc_anderson_opts;
+
+enum { ORC_MIX_PLAIN = 0, ORC_MIX_RELATIVE = 1, ORC_MIX_LOG = 2 };
+
+typedef struct orc_anderson_stats {
+  int iterations;   /* residual evaluations made */
+  int restarts;
+  double final_err;
+  double *err_trace; /* optional, trace_len entries */
+  int trace_len;
+} orc_anderson_stats;
+
+int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
+                 const orc_anderson_opts *o, const double *phi_t_unk,
+                 orc_anderson_stats *st);
+
+/* vector function for Broyden (nonzero return = the reference's funcerr) */
+typedef int (*orc_vecfunc_fn)(int n, const double *x, double *f, void *ctx);
+
+typedef struct orc_broydn_stats {
+  int iterations;
+  long evaluations;
+  double final_err;
+  double *err_trace;
+  int trace_len;
+} orc_broydn_stats;
+
+int orc_broydn(double *x, int n, orc_vecfunc_fn F, void *ctx, double tolf,
+               int *jc, double *qt, double *r, double *d,
+               orc_broydn_stats *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""scft_amd -- host-side mirror of libscft_hip.so, the MI355X (gfx950) SCFT field
+evaluation.
+
+The compute lives in the C ABI (include/scft.h); this module only loads the
+in-tree library, marshals numpy arrays through ctypes and mirrors the
+reference's operator interface for the path:
+
+* ``simple_FEM_1D_transient(w, out)`` -- the 1D_FEM.c:47-48 evaluator
+  (NR 1-based arrays, out = phi - phi_t);
+* ``Batch`` -- many independent instances evaluated / solved in one launch per
+  size bucket;
+* ``MLP`` -- the FP32 warm-start network with hand-written kernels.
+
+There is no fallback: if the shared library is missing the import raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libscft_hip.so")
+
+SCFT_OK = 0
+BC_DD, BC_DN = 0, 1
+C_ROWSCALED, C_CONSISTENT = 0, 1
+ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK = 1, 2, 4, 8, 16
+MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
+L_REF = 3.72374357332160
+TAU = 0.530252230020752
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class ScftError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("device", C.c_int), ("N", C.c_int), ("L", C.c_double), ("tau", C.c_double),
+                ("bc", C.c_int), ("cmat", C.c_int), ("beta", C.c_double), ("verbose", C.c_int)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("n_inst", C.c_int), ("M", C.POINTER(C.c_int)), ("N", C.POINTER(C.c_int)),
+                ("xoff", C.POINTER(C.c_longlong)), ("x", C.POINTER(C.c_double)),
+                ("phi_t", C.POINTER(C.c_double)), ("bc", C.c_int), ("cmat", C.c_int),
+                ("tune_file", C.c_char_p)]
+
+
+class AndersonOpts(C.Structure):
+    _fields_ = [("history", C.c_int), ("beta", C.c_double), ("tol", C.c_double),
+                ("max_iter", C.c_int), ("step_cap", C.c_double),
+                ("restart_factor", C.c_double), ("regularization", C.c_double),
+                ("transform", C.c_int), ("floor", C.c_double),
+                ("fallback_factor", C.c_double), ("check_every", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("converged", C.c_int), ("seconds", C.c_double),
+                ("eval_seconds", C.c_double), ("node_steps", C.c_double), ("bytes", C.c_double)]
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built: run `python scft_amd/build.py` (hipcc, gfx950)")
+    L = C.CDLL(LIB_PATH, mode=C.RTLD_GLOBAL)
+    L.scft_default_config.argtypes = [C.POINTER(Config)]
+    L.scft_init.argtypes = [C.POINTER(Config)]
+    L.scft_strerror.restype = C.c_char_p
+    L.scft_strerror.argtypes = [C.c_int]
+    L.simple_FEM_1D_transient.argtypes = [C.c_int, _dp, _dp]
+    L.scft_set_target.argtypes = [C.c_int, C.c_void_p]
+    L.scft_fixed_point.argtypes = [C.c_int, _dp, _dp]
+    L.scft_fixed_point.restype = None
+    L.scft_fixed_point_nr.argtypes = [C.c_int, _dp, _dp]
+    L.scft_fixed_point_nr.restype = None
+    L.scft_adm_chen_f.argtypes = [C.c_int, _dp, _dp]
+    L.scft_adm_chen_f.restype = None
+    L.scft_batch_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(BatchDesc)]
+    L.scft_batch_destroy.argtypes = [C.c_void_p]
+    L.scft_batch_layout.argtypes = [C.c_void_p, _lp, C.POINTER(C.c_longlong)]
+    L.scft_batch_eval.argtypes = [C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p,
+                                  C.c_void_p]
+    L.scft_default_anderson.argtypes = [C.POINTER(AndersonOpts)]
+    L.scft_default_anderson.restype = None
+    L.scft_batch_solve.argtypes = [C.c_void_p, C.POINTER(AndersonOpts), _dp, C.c_void_p,
+                                   C.c_void_p, C.c_void_p, C.POINTER(SolveStats)]
+    L.scft_batch_time_eval.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]
+    L.scft_batch_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.scft_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), _ip, _ip, _ip, C.c_int]
+    L.scft_debug_slabs.argtypes = [C.c_void_p, C.c_int, _dp]
+    return L
+
+
+lib = _load()
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != SCFT_OK:
+        raise ScftError(f"{what}: {lib.scft_strerror(rc).decode()} (code {rc})")
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+# ------------------------------------------------------------------ drop-in
+def init(N=2048, L=L_REF, tau=TAU, bc=BC_DD, cmat=C_ROWSCALED, beta=0.1, device=0,
+         verbose=False):
+    cfg = Config()
+    lib.scft_default_config(C.byref(cfg))
+    cfg.device, cfg.N, cfg.L, cfg.tau = device, N, L, tau
+    cfg.bc, cfg.cmat, cfg.beta, cfg.verbose = bc, cmat, beta, 1 if verbose else 0
+    check(lib.scft_init(C.byref(cfg)), "scft_init")
+
+
+def finalize():
+    check(lib.scft_finalize(), "scft_finalize")
+
+
+def set_target(phi_t_nodes):
+    if phi_t_nodes is None:
+        check(lib.scft_set_target(0, None), "scft_set_target")
+    else:
+        a = _f64(phi_t_nodes)
+        set_target._keep = a
+        check(lib.scft_set_target(len(a), a.ctypes.data), "scft_set_target")
+
+
+def simple_FEM_1D_transient(w_nr, out_nr):
+    """The reference evaluator signature (1D_FEM.c:47-48): w[1..n] -> out[1..n]."""
+    n = len(w_nr) - 1
+    return lib.simple_FEM_1D_transient(n, w_nr, out_nr)
+
+
+def funcerr() -> int:
+    return C.c_int.in_dll(lib, "funcerr").value
+
+
+def local_iteration() -> int:
+    return C.c_int.in_dll(lib, "local_iteration").value
+
+
+# ------------------------------------------------------------------ batches
+def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
+                  restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
+                  fallback_factor=0.0, check_every=8):
+    return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
+                        transform, floor, fallback_factor, check_every)
+
+
+class Batch:
+    """Independent SCFT instances on one device context.
+
+    xs: list of node-coordinate arrays (M_k each); Ns: contour steps; phits:
+    list of targets on each instance's unknowns.
+    """
+
+    def __init__(self, xs, Ns, phits, bc=BC_DN, cmat=C_ROWSCALED, tune_file=None):
+        self.bc, self.cmat = bc, cmat
+        self.M = np.array([len(x) for x in xs], dtype=np.int32)
+        self.N = np.ascontiguousarray(Ns, dtype=np.int32)
+        self.n = self.M - (2 if bc == BC_DD else 1)
+        xoff = np.zeros(len(xs), dtype=np.int64)
+        xoff[1:] = np.cumsum(self.M[:-1])
+        self._x = _f64(np.concatenate(xs))
+        self._pt = _f64(np.concatenate(phits))
+        self._xoff = xoff
+        d = BatchDesc(len(xs), self.M.ctypes.data_as(C.POINTER(C.c_int)),
+                      self.N.ctypes.data_as(C.POINTER(C.c_int)),
+                      xoff.ctypes.data_as(C.POINTER(C.c_longlong)),
+                      self._x.ctypes.data_as(C.POINTER(C.c_double)),
+                      self._pt.ctypes.data_as(C.POINTER(C.c_double)), bc, cmat,
+                      tune_file.encode() if tune_file else None)
+        h = C.c_void_p()
+        check(lib.scft_batch_create(C.byref(h), C.byref(d)), "scft_batch_create")
+        self.h = h
+        self.uoff = np.zeros(len(xs), dtype=np.int64)
+        tot = C.c_longlong()
+        check(lib.scft_batch_layout(self.h, self.uoff, C.byref(tot)), "scft_batch_layout")
+        self.total = tot.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.scft_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def split(self, flat):
+        return [flat[o:o + n] for o, n in zip(self.uoff, self.n)]
+
+    def join(self, parts):
+        return _f64(np.concatenate(parts))
+
+    def eval(self, w_parts):
+        w = self.join(w_parts)
+        phi = np.empty(self.total)
+        r = np.empty(self.total)
+        rn = np.empty(len(self.n))
+        st = np.zeros(len(self.n), dtype=np.uint32)
+        check(lib.scft_batch_eval(self.h, w, phi.ctypes.data, r.ctypes.data, rn.ctypes.data,
+                                  st.ctypes.data), "scft_batch_eval")
+        return dict(phi=self.split(phi), r=self.split(r), rnorm=rn, status=st)
+
+    def solve(self, w_parts, opts=None, w_cold_parts=None):
+        opts = opts or anderson_opts()
+        w = self.join(w_parts).copy()
+        wc = self.join(w_cold_parts) if w_cold_parts is not None else None
+        it = np.zeros(len(self.n), dtype=np.int32)
+        st = np.zeros(len(self.n), dtype=np.uint32)
+        stats = SolveStats()
+        rc = lib.scft_batch_solve(self.h, C.byref(opts), w, wc.ctypes.data if wc is not None
+                                  else None, it.ctypes.data, st.ctypes.data, C.byref(stats))
+        if rc not in (0, 7):
+            check(rc, "scft_batch_solve")
+        return dict(w=self.split(w), iters=it, status=st, sweeps=stats.sweeps,
+                    converged=stats.converged, seconds=stats.seconds,
+                    node_steps=stats.node_steps, bytes=stats.bytes, rc=rc)
+
+    def time_eval(self, w_parts, reps=5):
+        w = self.join(w_parts)
+        ms = C.c_double()
+        mm = C.c_double()
+        check(lib.scft_batch_time_eval(self.h, w, reps, C.byref(ms), C.byref(mm)),
+              "scft_batch_time_eval")
+        return ms.value
+
+    def autotune(self, reps=2):
+        check(lib.scft_batch_autotune(self.h, reps), "scft_batch_autotune")
+
+    def tuning(self):
+        nb = C.c_int()
+        nm = np.zeros(16, dtype=np.int32)
+        K = np.zeros(16, dtype=np.int32)
+        W = np.zeros(16, dtype=np.int32)
+        check(lib.scft_batch_tuning(self.h, C.byref(nb), nm, K, W, 16), "scft_batch_tuning")
+        return [(int(nm[i]), int(K[i]), int(W[i])) for i in range(nb.value)]
+
+    def debug_slabs(self, inst=0):
+        """Stored slabs q_1..q_{N/2-1} of a single-instance batch (node order)."""
+        n, N = int(self.n[inst]), int(self.N[inst])
+        out = np.empty((N // 2 - 1) * n)
+        check(lib.scft_debug_slabs(self.h, inst, out), "scft_debug_slabs")
+        return out.reshape(N // 2 - 1, n)

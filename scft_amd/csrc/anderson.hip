@@ -1,0 +1,281 @@
+// anderson.hip -- device-resident safeguarded Anderson mixing, one workgroup
+// per instance, one launch per sweep for the whole batch.
+//
+// Restates the update of adm.c:232-306 in residual form: with r_k the mixing
+// residual of w_k and r_{k-1..k-m} the stored ones,
+//   theta = argmin || r_k - sum_a theta_a (r_k - r_{k-a}) ||_2
+// from the Gram system U theta = b (adm.c:232-276) solved by Gauss-Jordan
+// with full pivoting (gaussj.c:7-60, singular -> history restart as
+// ADM_chen_C.c:103-112), then
+//   w_{k+1} = w_k - sum theta_a (w_k - w_{k-a}) + beta (r_k - sum theta_a (r_k - r_{k-a}))
+// (adm.c:289-306 with mixing beta).  Safeguards: the step is scaled to
+// ||w_{k+1} - w_k||_inf <= step_cap, the history is cleared when ||r_k||
+// exceeds restart_factor times the best so far, and a warm start whose first
+// residual is fallback_factor times the batch median is replaced by the cold
+// start.  Convergence is judged on the plain residual phi - phi_t.
+#include "scft_internal.h"
+
+namespace scft {
+
+constexpr int AM_THREADS = 256;
+constexpr int AM_MAXH = 12;  // previous iterates supported (history <= 12)
+
+__device__ __forceinline__ double mix_residual(double r, double pt, int transform,
+                                               double floor) {
+  if (transform == 0) return r;
+  const double p = fmax(pt, floor);
+  if (transform == 1) return r / p;
+  return log(fmax(p + r, floor * 1e-3) / p);
+}
+
+__global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
+  const int inst = blockIdx.x;
+  const int tid = threadIdx.x;
+  __shared__ double U[AM_MAXH * AM_MAXH];
+  __shared__ double bvec[AM_MAXH];
+  __shared__ double red[AM_THREADS / 64][AM_MAXH * (AM_MAXH + 1) / 2 + AM_MAXH];
+  __shared__ double red1[AM_THREADS / 64];
+  __shared__ int s_m, s_mode;
+  __shared__ double s_scale;
+
+  if (__builtin_amdgcn_readfirstlane(a.status[inst]) & ST_DONE_MASK) return;
+  const InstDesc d = a.desc[inst];
+  const int n = __builtin_amdgcn_readfirstlane(d.n);
+  const int S = a.history + 1;
+  double* __restrict__ w = a.w + d.uo;
+  const double* __restrict__ r = a.r + d.uo;
+  const double* __restrict__ pt = a.phit + d.uo;
+  double* __restrict__ hw = a.hw + d.ho;
+  double* __restrict__ hr = a.hr + d.ho;
+  MixState ms = a.st[inst];
+  const double err = a.rnorm[inst];
+  const int k = ms.k + 1;  // evaluations including this one
+
+  // ------------------------------------------------------------ stop tests
+  if (tid == 0) {
+    s_mode = 0;
+    if (!(err == err)) {
+      s_mode = 1;  // NaN already flagged by the march
+    } else if (err <= a.tol) {
+      s_mode = 2;
+    } else if (ms.k == 0 && a.fallback_thresh > 0.0 && err > a.fallback_thresh &&
+               a.wcold != nullptr) {
+      s_mode = 3;  // reject the warm start
+    } else if (k >= a.max_iter) {
+      s_mode = 4;
+    }
+  }
+  __syncthreads();
+  const int mode = __builtin_amdgcn_readfirstlane(s_mode);
+  if (mode == 1 || mode == 2 || mode == 4) {
+    if (tid == 0) {
+      if (mode == 2) a.status[inst] |= ST_CONVERGED;
+      if (mode == 4) a.status[inst] |= ST_MAXITER;
+      a.iters[inst] = k;
+      ms.k = k;
+      a.st[inst] = ms;
+    }
+    return;
+  }
+  if (mode == 3) {  // restart from the cold-start field, history empty
+    const double* wc = a.wcold + d.uo;
+    for (int u = tid; u < n; u += AM_THREADS) w[u] = wc[u];
+    if (tid == 0) {
+      a.status[inst] |= ST_FALLBACK;
+      ms.k = k;  // the rejected evaluation counts
+      ms.stored = 0;
+      ms.head = 0;
+      ms.best = INFINITY;
+      a.st[inst] = ms;
+      atomicAdd(a.active, 1);
+    }
+    return;
+  }
+
+  // ----------------------------------------------- history bookkeeping
+  int stored = ms.stored;
+  if (err > a.restart_factor * ms.best && stored > 0) {
+    stored = 0;
+    ms.restarts++;
+  }
+  if (err < ms.best) ms.best = err;
+  const int cur = __builtin_amdgcn_readfirstlane(ms.head);
+  int m = __builtin_amdgcn_readfirstlane(stored < a.history ? stored : a.history);
+
+  // push (w_k, rt_k) into slot `cur`
+  for (int u = tid; u < n; u += AM_THREADS) {
+    hw[(long long)cur * n + u] = w[u];
+    hr[(long long)cur * n + u] = mix_residual(r[u], pt[u], a.transform, a.floor);
+  }
+  __syncthreads();
+
+  // ----------------------------------------------------- Gram system
+  if (m > 0) {
+    constexpr int NP = AM_MAXH * (AM_MAXH + 1) / 2 + AM_MAXH;
+    double part[NP];
+#pragma unroll
+    for (int e = 0; e < NP; e++) part[e] = 0.0;
+    int slot[AM_MAXH];
+#pragma unroll
+    for (int i = 0; i < AM_MAXH; i++) slot[i] = ((cur - (i + 1)) % S + S) % S;
+    for (int u = tid; u < n; u += AM_THREADS) {
+      const double rk = hr[(long long)cur * n + u];
+      double dv[AM_MAXH];
+#pragma unroll
+      for (int i = 0; i < AM_MAXH; i++)
+        dv[i] = (i < m) ? rk - hr[(long long)slot[i] * n + u] : 0.0;
+      int e = 0;
+#pragma unroll
+      for (int i = 0; i < AM_MAXH; i++) {
+#pragma unroll
+        for (int j = 0; j <= i; j++) part[e++] += dv[i] * dv[j];
+      }
+#pragma unroll
+      for (int i = 0; i < AM_MAXH; i++) part[e++] += dv[i] * rk;
+    }
+    const int lane = tid & 63, wave = tid >> 6;
+#pragma unroll
+    for (int e = 0; e < NP; e++) {
+      double v = part[e];
+#pragma unroll
+      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+      if (lane == 0) red[wave][e] = v;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double tot[NP];
+      for (int e = 0; e < NP; e++) {
+        double s = 0.0;
+        for (int v = 0; v < AM_THREADS / 64; v++) s += red[v][e];
+        tot[e] = s;
+      }
+      int e = 0;
+      for (int i = 0; i < AM_MAXH; i++)
+        for (int j = 0; j <= i; j++) {
+          if (i < m && j < m) U[i * m + j] = U[j * m + i] = tot[e];
+          e++;
+        }
+      for (int i = 0; i < AM_MAXH; i++) {
+        if (i < m) bvec[i] = tot[e];
+        e++;
+      }
+      double tr = 0.0;
+      for (int i = 0; i < m; i++) tr += U[i * m + i];
+      for (int i = 0; i < m; i++) U[i * m + i] += a.reg * tr;
+      // Gauss-Jordan with full pivoting (gaussj.c:7-60); zero pivot = singular
+      int ipiv[AM_MAXH], indxr[AM_MAXH], indxc[AM_MAXH];
+      bool sing = !(tr > 0.0);
+      for (int j = 0; j < m; j++) ipiv[j] = 0;
+      for (int i = 0; i < m && !sing; i++) {
+        double big = 0.0;
+        int irow = 0, icol = 0;
+        for (int j = 0; j < m; j++)
+          if (ipiv[j] != 1)
+            for (int kk = 0; kk < m; kk++)
+              if (ipiv[kk] == 0 && fabs(U[j * m + kk]) >= big) {
+                big = fabs(U[j * m + kk]);
+                irow = j;
+                icol = kk;
+              }
+        ++ipiv[icol];
+        if (irow != icol) {
+          for (int l = 0; l < m; l++) {
+            double t = U[irow * m + l];
+            U[irow * m + l] = U[icol * m + l];
+            U[icol * m + l] = t;
+          }
+          double t = bvec[irow];
+          bvec[irow] = bvec[icol];
+          bvec[icol] = t;
+        }
+        indxr[i] = irow;
+        indxc[i] = icol;
+        if (U[icol * m + icol] == 0.0) {
+          sing = true;
+          break;
+        }
+        const double pivinv = 1.0 / U[icol * m + icol];
+        U[icol * m + icol] = 1.0;
+        for (int l = 0; l < m; l++) U[icol * m + l] *= pivinv;
+        bvec[icol] *= pivinv;
+        for (int ll = 0; ll < m; ll++)
+          if (ll != icol) {
+            const double dum = U[ll * m + icol];
+            U[ll * m + icol] = 0.0;
+            for (int l = 0; l < m; l++) U[ll * m + l] -= U[icol * m + l] * dum;
+            bvec[ll] -= bvec[icol] * dum;
+          }
+      }
+      (void)indxr;
+      (void)indxc;
+      for (int i = 0; i < m && !sing; i++)
+        if (!isfinite(bvec[i])) sing = true;
+      s_m = sing ? 0 : m;
+      if (sing) ms.restarts++;
+    }
+    __syncthreads();
+    const int sm = __builtin_amdgcn_readfirstlane(s_m);
+    if (sm == 0) stored = 0;  // keep only the current pair
+    m = sm;
+  }
+
+  // ----------------------------------------------------------- update
+  double stepmax = 0.0;
+  for (int u = tid; u < n; u += AM_THREADS) {
+    const double wk = w[u];
+    const double rk = hr[(long long)cur * n + u];
+    double wb = wk, rb = rk;
+    for (int i = 0; i < m; i++) {
+      const int sl = ((cur - (i + 1)) % S + S) % S;
+      wb -= bvec[i] * (wk - hw[(long long)sl * n + u]);
+      rb -= bvec[i] * (rk - hr[(long long)sl * n + u]);
+    }
+    const double wn = wb + a.beta * rb;
+    stepmax = fmax(stepmax, fabs(wn - wk));
+    w[u] = wn;  // provisional, rescaled below
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) stepmax = fmax(stepmax, __shfl_xor(stepmax, off));
+  if ((tid & 63) == 0) red1[tid >> 6] = stepmax;
+  __syncthreads();
+  if (tid == 0) {
+    double s = 0.0;
+    for (int v = 0; v < AM_THREADS / 64; v++) s = fmax(s, red1[v]);
+    s_scale = (a.step_cap > 0.0 && s > a.step_cap) ? a.step_cap / s : 1.0;
+  }
+  __syncthreads();
+  const double scale = s_scale;
+  if (scale != 1.0) {
+    for (int u = tid; u < n; u += AM_THREADS) {
+      const double wk = hw[(long long)cur * n + u];
+      w[u] = wk + scale * (w[u] - wk);
+    }
+  }
+  if (tid == 0) {
+    ms.k = k;
+    ms.head = (cur + 1) % S;
+    ms.stored = (stored < S) ? stored + 1 : S;
+    a.st[inst] = ms;
+    atomicAdd(a.active, 1);
+  }
+}
+
+// zero the per-sweep counters (bucket queues and the active count) with a
+// kernel rather than a memset node, so the sweep can be captured in a graph
+__global__ void reset_counters_kernel(int* counters, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) counters[i] = 0;
+}
+
+hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(anderson_kernel, dim3(a.n_inst), dim3(AM_THREADS), 0, s, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_reset(int* counters, int n, hipStream_t s) {
+  hipLaunchKernelGGL(reset_counters_kernel, dim3((n + 255) / 256), dim3(256), 0, s, counters,
+                     n);
+  return hipGetLastError();
+}
+
+}  // namespace scft

@@ -1,0 +1,829 @@
+// capi.hip -- host side of libscft_hip.so: batched contexts, the size-bucket
+// scheduler and autotuner, the graph-captured solve loop, and the reference
+// drop-in entry points.  Everything exported here is declared in scft.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/scft.h"
+#include "host_math.h"
+#include "scft_internal.h"
+
+using namespace scft;
+
+// ------------------------------------------------------------ error plumbing
+static thread_local char g_errmsg[512];
+
+#define HIPCHK(expr)                                                          \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess) {                                                   \
+      snprintf(g_errmsg, sizeof g_errmsg, "%s: %s (%s:%d)", #expr,            \
+               hipGetErrorString(e_), __FILE__, __LINE__);                    \
+      return e_ == hipErrorOutOfMemory ? SCFT_E_NOMEM : SCFT_E_HIP;           \
+    }                                                                         \
+  } while (0)
+
+#define SCFT_TRY(expr)        \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != SCFT_OK) return rc_; \
+  } while (0)
+
+template <typename T>
+static int dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)));
+  return SCFT_OK;
+}
+
+template <typename T>
+static void dfree(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// ------------------------------------------------------------------ buckets
+namespace {
+
+struct Bucket {
+  int cap = 0;  // 64 * W * K
+  int K = 1, W = 1;
+  int nmax = 0, Nmax = 0;
+  std::vector<int> items;  // instance ids, largest cost first
+  int* d_items = nullptr;
+  int grid = 0;
+  long long slab_stride = 0;  // doubles per workgroup
+};
+
+int default_K(int cap) { return std::min(cap / 64, 16); }
+
+int cap_for(int n) {
+  int c = 64;
+  while (c < n) c *= 2;
+  return c;
+}
+
+}  // namespace
+
+struct scft_batch {
+  int device = 0;
+  int n_inst = 0;
+  int bc = SCFT_BC_DD, cmat = SCFT_C_ROWSCALED;
+  std::vector<InstDesc> desc;
+  std::vector<long long> uoff;
+  long long total_u = 0, total_x = 0, total_pw = 0, total_h = 0;
+  std::string tune_file;
+  std::vector<Bucket> buckets;
+  hipStream_t stream = nullptr;
+  // device state
+  InstDesc* d_desc = nullptr;
+  double *d_x = nullptr, *d_w = nullptr, *d_phit = nullptr, *d_pw = nullptr;
+  double *d_phi = nullptr, *d_r = nullptr, *d_rnorm = nullptr;
+  unsigned* d_status = nullptr;
+  double* d_slabs = nullptr;
+  long long slab_doubles = 0;
+  int* d_counters = nullptr;  // bucket queues, then the active count
+  // mixer
+  int hist_S = 0;
+  double *d_hw = nullptr, *d_hr = nullptr, *d_wcold = nullptr;
+  MixState* d_ms = nullptr;
+  int* d_iters = nullptr;
+  // algorithmic work per full sweep (all instances)
+  double node_steps_all = 0.0, bytes_all = 0.0;
+};
+
+// Algorithmic HBM bytes of one evaluation: q-slabs 1..N/2-1 written once and
+// read back once, plus the O(n) inputs/outputs (x, w, phi_t in; phi, r out).
+static double eval_bytes(int n, int N) {
+  return 8.0 * n * (2.0 * (N / 2 - 1)) + 40.0 * n;
+}
+
+static int size_scratch(scft_batch* b) {
+  long long need = 0;
+  for (auto& bk : b->buckets) {
+    if (bk.items.empty()) continue;
+    if (!march_variant_exists(bk.K, bk.W)) {
+      snprintf(g_errmsg, sizeof g_errmsg, "no march variant K=%d W=%d", bk.K, bk.W);
+      return SCFT_E_ARG;
+    }
+    const int P = 64 * bk.W;
+    const int mg = march_max_grid(bk.K, bk.W);
+    bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), mg));
+    bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2 - 1)) * P * bk.K;
+    need = std::max(need, bk.slab_stride * bk.grid);
+  }
+  if (need > b->slab_doubles) {
+    dfree(b->d_slabs);
+    SCFT_TRY(dalloc(&b->d_slabs, static_cast<size_t>(need)));
+    b->slab_doubles = need;
+  }
+  return SCFT_OK;
+}
+
+static void read_tune_file(scft_batch* b) {
+  if (b->tune_file.empty()) return;
+  std::ifstream f(b->tune_file);
+  int cap, K, W;
+  while (f >> cap >> K >> W) {
+    for (auto& bk : b->buckets)
+      if (bk.cap == cap && 64 * K * W >= cap && march_variant_exists(K, W)) {
+        bk.K = K;
+        bk.W = W;
+      }
+  }
+}
+
+static void write_tune_file(const scft_batch* b) {
+  if (b->tune_file.empty()) return;
+  std::ofstream f(b->tune_file);
+  for (auto& bk : b->buckets)
+    if (!bk.items.empty()) f << bk.cap << " " << bk.K << " " << bk.W << "\n";
+}
+
+static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_idx) {
+  MarchArgs a;
+  a.desc = b->d_desc;
+  a.x = b->d_x;
+  a.w = b->d_w;
+  a.phit = b->d_phit;
+  a.pw = b->d_pw;
+  a.phi = b->d_phi;
+  a.r = b->d_r;
+  a.rnorm = b->d_rnorm;
+  a.status = b->d_status;
+  a.slabs = b->d_slabs;
+  a.slab_stride = bk.slab_stride;
+  a.items = bk.d_items;
+  a.n_items = static_cast<int>(bk.items.size());
+  (void)bucket_idx;
+  return a;
+}
+
+// One evaluation sweep of every instance not marked done: counters reset,
+// then one persistent launch per non-empty bucket, largest bucket first.
+static int enqueue_sweep(scft_batch* b) {
+  const int nb = static_cast<int>(b->buckets.size());
+  HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+  for (int i = nb - 1; i >= 0; i--) {
+    const Bucket& bk = b->buckets[i];
+    if (bk.items.empty()) continue;
+    HIPCHK(launch_march(bk.K, bk.W, march_args(b, bk, i), bk.grid, b->stream));
+  }
+  return SCFT_OK;
+}
+
+// ------------------------------------------------------------- public: batch
+extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
+  if (!out || !dsc || dsc->n_inst <= 0 || !dsc->M || !dsc->N || !dsc->xoff || !dsc->x ||
+      !dsc->phi_t)
+    return SCFT_E_ARG;
+  if (dsc->bc != SCFT_BC_DD && dsc->bc != SCFT_BC_DN) return SCFT_E_ARG;
+  if (dsc->cmat != SCFT_C_ROWSCALED && dsc->cmat != SCFT_C_CONSISTENT) return SCFT_E_ARG;
+  *out = nullptr;
+  auto* b = new scft_batch();
+  b->n_inst = dsc->n_inst;
+  b->bc = dsc->bc;
+  b->cmat = dsc->cmat;
+  if (dsc->tune_file) b->tune_file = dsc->tune_file;
+  if (hipGetDevice(&b->device) != hipSuccess) {
+    delete b;
+    return SCFT_E_HIP;
+  }
+  // descriptors, offsets, pairing weights per distinct N
+  std::vector<int> Ns;
+  std::vector<std::vector<double>> pws;
+  std::vector<long long> pwo;
+  long long pw_total = 0, x_total = 0;
+  b->desc.resize(b->n_inst);
+  b->uoff.resize(b->n_inst);
+  for (int k = 0; k < b->n_inst; k++) {
+    const int M = dsc->M[k], N = dsc->N[k];
+    if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > 4096 || N < 16 || (N & (N - 1)) != 0) {
+      snprintf(g_errmsg, sizeof g_errmsg, "instance %d: M=%d N=%d unsupported", k, M, N);
+      delete b;
+      return SCFT_E_ARG;
+    }
+    InstDesc& d = b->desc[k];
+    d.M = M;
+    d.N = N;
+    d.n = (b->bc == SCFT_BC_DD) ? M - 2 : M - 1;
+    d.bc = b->bc;
+    d.cmat = b->cmat;
+    d.pad = 0;
+    d.xo = dsc->xoff[k];
+    d.uo = b->total_u;
+    d.wall0 = 0.0;
+    d.wallL = 0.0;
+    b->uoff[k] = b->total_u;
+    b->total_u += d.n;
+    x_total = std::max<long long>(x_total, dsc->xoff[k] + M);
+    auto it = std::find(Ns.begin(), Ns.end(), N);
+    if (it == Ns.end()) {
+      std::vector<double> pw;
+      pairing_weights(N, pw);
+      Ns.push_back(N);
+      pwo.push_back(pw_total);
+      pw_total += static_cast<long long>(pw.size());
+      pws.push_back(pw);
+      d.po = pwo.back();
+    } else {
+      d.po = pwo[it - Ns.begin()];
+    }
+    d.ho = 0;
+    b->node_steps_all += static_cast<double>(d.n) * N;
+    b->bytes_all += eval_bytes(d.n, N);
+  }
+  b->total_x = x_total;
+  b->total_pw = pw_total;
+  // buckets by power-of-two capacity, items sorted by cost n*N descending
+  std::vector<int> order(b->n_inst);
+  for (int k = 0; k < b->n_inst; k++) order[k] = k;
+  std::sort(order.begin(), order.end(), [&](int i, int j) {
+    const double ci = static_cast<double>(b->desc[i].n) * b->desc[i].N;
+    const double cj = static_cast<double>(b->desc[j].n) * b->desc[j].N;
+    return ci > cj || (ci == cj && i < j);
+  });
+  for (int cap = 64; cap <= 4096; cap *= 2) {
+    Bucket bk;
+    bk.cap = cap;
+    bk.K = default_K(cap);
+    bk.W = cap / 64 / bk.K;
+    b->buckets.push_back(bk);
+  }
+  for (int k : order) {
+    const int c = cap_for(b->desc[k].n);
+    for (auto& bk : b->buckets)
+      if (bk.cap == c) {
+        bk.items.push_back(k);
+        bk.nmax = std::max(bk.nmax, b->desc[k].n);
+        bk.Nmax = std::max(bk.Nmax, b->desc[k].N);
+      }
+  }
+  read_tune_file(b);
+
+  int rc = SCFT_OK;
+  auto fail = [&](int code) {
+    scft_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(SCFT_E_HIP);
+  if ((rc = dalloc(&b->d_desc, b->n_inst)) || (rc = dalloc(&b->d_x, b->total_x)) ||
+      (rc = dalloc(&b->d_w, b->total_u)) || (rc = dalloc(&b->d_phit, b->total_u)) ||
+      (rc = dalloc(&b->d_pw, b->total_pw)) || (rc = dalloc(&b->d_phi, b->total_u)) ||
+      (rc = dalloc(&b->d_r, b->total_u)) || (rc = dalloc(&b->d_rnorm, b->n_inst)) ||
+      (rc = dalloc(&b->d_status, b->n_inst)) ||
+      (rc = dalloc(&b->d_counters, b->buckets.size() + 1)))
+    return fail(rc);
+  for (auto& bk : b->buckets) {
+    if (bk.items.empty()) continue;
+    if ((rc = dalloc(&bk.d_items, bk.items.size()))) return fail(rc);
+    if (hipMemcpy(bk.d_items, bk.items.data(), bk.items.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      return fail(SCFT_E_HIP);
+  }
+  std::vector<double> pw_all(std::max<long long>(1, b->total_pw));
+  for (size_t i = 0; i < Ns.size(); i++)
+    std::copy(pws[i].begin(), pws[i].end(), pw_all.begin() + pwo[i]);
+  if (hipMemcpy(b->d_desc, b->desc.data(), sizeof(InstDesc) * b->n_inst,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_x, dsc->x, sizeof(double) * b->total_x, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_phit, dsc->phi_t, sizeof(double) * b->total_u, hipMemcpyHostToDevice) !=
+          hipSuccess ||
+      hipMemcpy(b->d_pw, pw_all.data(), sizeof(double) * b->total_pw,
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemset(b->d_status, 0, sizeof(unsigned) * b->n_inst) != hipSuccess)
+    return fail(SCFT_E_HIP);
+  if ((rc = size_scratch(b))) return fail(rc);
+  *out = b;
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_destroy(scft_batch* b) {
+  if (!b) return SCFT_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  for (auto& bk : b->buckets) dfree(bk.d_items);
+  dfree(b->d_desc);
+  dfree(b->d_x);
+  dfree(b->d_w);
+  dfree(b->d_phit);
+  dfree(b->d_pw);
+  dfree(b->d_phi);
+  dfree(b->d_r);
+  dfree(b->d_rnorm);
+  dfree(b->d_status);
+  dfree(b->d_slabs);
+  dfree(b->d_counters);
+  dfree(b->d_hw);
+  dfree(b->d_hr);
+  dfree(b->d_wcold);
+  dfree(b->d_ms);
+  dfree(b->d_iters);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_layout(const scft_batch* b, long long* uoff, long long* total) {
+  if (!b) return SCFT_E_STATE;
+  if (uoff) std::copy(b->uoff.begin(), b->uoff.end(), uoff);
+  if (total) *total = b->total_u;
+  return SCFT_OK;
+}
+
+// Replace the target of every instance (host array on unknowns).
+static int batch_set_target(scft_batch* b, const double* phit) {
+  HIPCHK(hipMemcpyAsync(b->d_phit, phit, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
+                        b->stream));
+  return SCFT_OK;
+}
+
+static int batch_eval_impl(scft_batch* b, const double* w, double* phi, double* r,
+                           double* rnorm, unsigned* status) {
+  HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  SCFT_TRY(enqueue_sweep(b));
+  if (phi)
+    HIPCHK(hipMemcpyAsync(phi, b->d_phi, sizeof(double) * b->total_u, hipMemcpyDeviceToHost,
+                          b->stream));
+  if (r)
+    HIPCHK(hipMemcpyAsync(r, b->d_r, sizeof(double) * b->total_u, hipMemcpyDeviceToHost,
+                          b->stream));
+  if (rnorm)
+    HIPCHK(hipMemcpyAsync(rnorm, b->d_rnorm, sizeof(double) * b->n_inst, hipMemcpyDeviceToHost,
+                          b->stream));
+  if (status)
+    HIPCHK(hipMemcpyAsync(status, b->d_status, sizeof(unsigned) * b->n_inst,
+                          hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_eval(scft_batch* b, const double* w, double* phi, double* r,
+                               double* rnorm, unsigned* status) {
+  if (!b || !w) return SCFT_E_ARG;
+  return batch_eval_impl(b, w, phi, r, rnorm, status);
+}
+
+extern "C" int scft_batch_time_eval(scft_batch* b, const double* w, int reps,
+                                    double* ms_per_eval, double* march_ms) {
+  if (!b || !w || reps < 1) return SCFT_E_ARG;
+  HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  SCFT_TRY(enqueue_sweep(b));  // warm-up
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, b->stream));
+  for (int i = 0; i < reps; i++) SCFT_TRY(enqueue_sweep(b));
+  HIPCHK(hipEventRecord(e1, b->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ms_per_eval) *ms_per_eval = ms / reps;
+  if (march_ms) *march_ms = ms / reps;
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
+  if (!b) return SCFT_E_STATE;
+  if (reps < 1) reps = 1;
+  const int Ks[] = {1, 2, 4, 8, 16};
+  const int Wsz[] = {1, 2, 4, 8, 16};
+  // current field: leave d_w as is (callers upload first); clear statuses
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  const int nb = static_cast<int>(b->buckets.size());
+  for (int bi = 0; bi < nb; bi++) {
+    Bucket& bk = b->buckets[bi];
+    if (bk.items.empty()) continue;
+    float best = 1e30f;
+    int bestK = bk.K, bestW = bk.W;
+    for (int K : Ks)
+      for (int W : Wsz) {
+        if (64 * K * W < bk.nmax || 64 * K * W > 2 * bk.cap) continue;
+        if (!march_variant_exists(K, W)) continue;
+        bk.K = K;
+        bk.W = W;
+        SCFT_TRY(size_scratch(b));
+        MarchArgs a = march_args(b, bk, bi);
+        HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+        HIPCHK(launch_march(K, W, a, bk.grid, b->stream));  // warm-up
+        HIPCHK(hipEventRecord(e0, b->stream));
+        for (int r = 0; r < reps; r++) {
+          HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+          HIPCHK(launch_march(K, W, a, bk.grid, b->stream));
+        }
+        HIPCHK(hipEventRecord(e1, b->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+          best = ms;
+          bestK = K;
+          bestW = W;
+        }
+      }
+    bk.K = bestK;
+    bk.W = bestW;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SCFT_TRY(size_scratch(b));
+  write_tune_file(b);
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_tuning(const scft_batch* b, int* n_buckets, int* bucket_nmax, int* K,
+                                 int* W, int max_buckets) {
+  if (!b) return SCFT_E_STATE;
+  int c = 0;
+  for (auto& bk : b->buckets) {
+    if (bk.items.empty()) continue;
+    if (c < max_buckets) {
+      if (bucket_nmax) bucket_nmax[c] = bk.nmax;
+      if (K) K[c] = bk.K;
+      if (W) W[c] = bk.W;
+    }
+    c++;
+  }
+  if (n_buckets) *n_buckets = c;
+  return SCFT_OK;
+}
+
+// Diagnostic: the q-slabs 1..N/2-1 left in the scratch by the last
+// evaluation of a single-instance batch, returned in node order
+// (out[(s-1)*n + u] = q_s at unknown u).
+extern "C" int scft_debug_slabs(scft_batch* b, int inst, double* out) {
+  if (!b || !out || b->n_inst != 1 || inst != 0) return SCFT_E_ARG;
+  const InstDesc& d = b->desc[0];
+  for (auto& bk : b->buckets) {
+    if (bk.items.empty()) continue;
+    const int P = 64 * bk.W, K = bk.K, half = d.N / 2;
+    std::vector<double> raw(static_cast<size_t>(half - 1) * P * K);
+    HIPCHK(hipMemcpy(raw.data(), b->d_slabs, raw.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int s = 1; s < half; s++)
+      for (int u = 0; u < d.n; u++) {
+        const int p = u / K, t = u % K;
+        out[static_cast<size_t>(s - 1) * d.n + u] =
+            raw[static_cast<size_t>(s - 1) * P * K + static_cast<size_t>(t) * P + p];
+      }
+    return SCFT_OK;
+  }
+  return SCFT_E_STATE;
+}
+
+// ------------------------------------------------------------ Anderson solve
+extern "C" void scft_default_anderson(scft_anderson_opts* o) {
+  o->history = 10;
+  o->beta = 0.1;
+  o->tol = 1e-6;
+  o->max_iter = 5000;
+  o->step_cap = 0.5;
+  o->restart_factor = 1e30;
+  o->regularization = 1e-14;
+  o->transform = 0;
+  o->floor = 1e-8;
+  o->fallback_factor = 0.0;
+  o->check_every = 8;
+}
+
+static int ensure_mixer(scft_batch* b, int S) {
+  if (b->hist_S == S && b->d_hw) return SCFT_OK;
+  dfree(b->d_hw);
+  dfree(b->d_hr);
+  dfree(b->d_ms);
+  dfree(b->d_iters);
+  long long off = 0;
+  for (auto& d : b->desc) {
+    d.ho = off;
+    off += static_cast<long long>(S) * d.n;
+  }
+  b->total_h = off;
+  SCFT_TRY(dalloc(&b->d_hw, off));
+  SCFT_TRY(dalloc(&b->d_hr, off));
+  SCFT_TRY(dalloc(&b->d_ms, b->n_inst));
+  SCFT_TRY(dalloc(&b->d_iters, b->n_inst));
+  HIPCHK(hipMemcpy(b->d_desc, b->desc.data(), sizeof(InstDesc) * b->n_inst,
+                   hipMemcpyHostToDevice));
+  b->hist_S = S;
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, double* w,
+                                const double* w_cold, int* iters, unsigned* status,
+                                scft_solve_stats* stats) {
+  if (!b || !o || !w) return SCFT_E_ARG;
+  if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
+  const auto t0 = std::chrono::steady_clock::now();
+  SCFT_TRY(ensure_mixer(b, o->history + 1));
+  if (w_cold && !b->d_wcold) SCFT_TRY(dalloc(&b->d_wcold, b->total_u));
+  const int nb = static_cast<int>(b->buckets.size());
+  HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
+                        b->stream));
+  if (w_cold)
+    HIPCHK(hipMemcpyAsync(b->d_wcold, w_cold, sizeof(double) * b->total_u,
+                          hipMemcpyHostToDevice, b->stream));
+  std::vector<MixState> ms0(b->n_inst, MixState{0, 0, 0, 0, INFINITY});
+  HIPCHK(hipMemcpyAsync(b->d_ms, ms0.data(), sizeof(MixState) * b->n_inst,
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  HIPCHK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * b->n_inst, b->stream));
+
+  AndersonArgs a;
+  a.desc = b->d_desc;
+  a.w = b->d_w;
+  a.r = b->d_r;
+  a.rnorm = b->d_rnorm;
+  a.phit = b->d_phit;
+  a.wcold = w_cold ? b->d_wcold : nullptr;
+  a.hw = b->d_hw;
+  a.hr = b->d_hr;
+  a.st = b->d_ms;
+  a.status = b->d_status;
+  a.iters = b->d_iters;
+  a.active = b->d_counters + nb;
+  a.n_inst = b->n_inst;
+  a.history = o->history;
+  a.transform = o->transform;
+  a.max_iter = o->max_iter;
+  a.beta = o->beta;
+  a.tol = o->tol;
+  a.step_cap = o->step_cap;
+  a.restart_factor = o->restart_factor;
+  a.reg = o->regularization;
+  a.floor = o->floor;
+  a.fallback_thresh = 0.0;
+
+  // first sweep eagerly: the warm-start fallback needs the median first residual
+  SCFT_TRY(enqueue_sweep(b));
+  if (w_cold && o->fallback_factor > 0.0) {
+    std::vector<double> rn(b->n_inst);
+    HIPCHK(hipMemcpyAsync(rn.data(), b->d_rnorm, sizeof(double) * b->n_inst,
+                          hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    std::vector<double> fin;
+    for (double v : rn)
+      if (std::isfinite(v)) fin.push_back(v);
+    if (!fin.empty()) {
+      std::nth_element(fin.begin(), fin.begin() + fin.size() / 2, fin.end());
+      a.fallback_thresh = o->fallback_factor * fin[fin.size() / 2];
+    }
+  }
+  HIPCHK(launch_anderson(a, b->stream));
+  a.fallback_thresh = 0.0;  // only the first evaluation may be rejected
+
+  // remaining sweeps: [reset, march per bucket, mixer] captured once
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = enqueue_sweep(b);
+  if (rc == SCFT_OK && launch_anderson(a, b->stream) != hipSuccess) rc = SCFT_E_HIP;
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (rc != SCFT_OK) return rc;
+  HIPCHK(ce);
+  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  int* h_active = nullptr;
+  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_active), sizeof(int)));
+  int sweeps = 1, check = std::max(1, o->check_every);
+  for (;;) {
+    HIPCHK(hipMemcpyAsync(h_active, b->d_counters + nb, sizeof(int), hipMemcpyDeviceToHost,
+                          b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (*h_active == 0 || sweeps >= o->max_iter) break;
+    for (int i = 0; i < check && sweeps < o->max_iter; i++, sweeps++)
+      HIPCHK(hipGraphLaunch(exec, b->stream));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  (void)hipHostFree(h_active);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+
+  HIPCHK(hipMemcpyAsync(w, b->d_w, sizeof(double) * b->total_u, hipMemcpyDeviceToHost,
+                        b->stream));
+  std::vector<int> it(b->n_inst);
+  std::vector<unsigned> stv(b->n_inst);
+  HIPCHK(hipMemcpyAsync(it.data(), b->d_iters, sizeof(int) * b->n_inst, hipMemcpyDeviceToHost,
+                        b->stream));
+  HIPCHK(hipMemcpyAsync(stv.data(), b->d_status, sizeof(unsigned) * b->n_inst,
+                        hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  int conv = 0;
+  double ns = 0.0, by = 0.0;
+  for (int k = 0; k < b->n_inst; k++) {
+    if (stv[k] & ST_CONVERGED) conv++;
+    ns += static_cast<double>(it[k]) * b->desc[k].n * b->desc[k].N;
+    by += it[k] * eval_bytes(b->desc[k].n, b->desc[k].N);
+  }
+  if (iters) std::copy(it.begin(), it.end(), iters);
+  if (status) std::copy(stv.begin(), stv.end(), status);
+  if (stats) {
+    stats->sweeps = sweeps;
+    stats->converged = conv;
+    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    stats->eval_seconds = 0.0;
+    stats->node_steps = ns;
+    stats->bytes = by;
+  }
+  return conv == b->n_inst ? SCFT_OK : SCFT_E_NOCONV;
+}
+
+// ------------------------------------------------------- drop-in evaluator
+extern "C" {
+double* f0_given = nullptr;
+int local_iteration = 0;
+int funcerr = 0;
+}
+
+namespace {
+std::mutex g_mu;
+bool g_init = false;
+scft_config g_cfg;
+scft_batch* g_one = nullptr;  // single-instance context of the drop-in entries
+int g_one_n = -1;
+std::vector<double> g_target;  // from scft_set_target (all nodes)
+std::vector<double> g_x, g_ptall, g_ptu, g_r;
+int g_last_status = SCFT_OK;
+}  // namespace
+
+extern "C" void scft_default_config(scft_config* c) {
+  c->device = 0;
+  c->N = 2048;
+  c->L = 3.72374357332160;
+  c->tau = 0.530252230020752;
+  c->bc = SCFT_BC_DD;
+  c->cmat = SCFT_C_ROWSCALED;
+  c->beta = 0.1;
+  c->verbose = 0;
+}
+
+static int init_locked(const scft_config* cfg) {
+  if (cfg) {
+    if (cfg->N < 16 || (cfg->N & (cfg->N - 1)) != 0 || !(cfg->L > 0.0) ||
+        (cfg->bc != SCFT_BC_DD && cfg->bc != SCFT_BC_DN))
+      return SCFT_E_ARG;
+    g_cfg = *cfg;
+  } else {
+    scft_default_config(&g_cfg);
+  }
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (g_cfg.device < 0 || g_cfg.device >= ndev) return SCFT_E_HIP;
+  HIPCHK(hipSetDevice(g_cfg.device));
+  if (g_one) scft_batch_destroy(g_one);
+  g_one = nullptr;
+  g_one_n = -1;
+  g_init = true;
+  return SCFT_OK;
+}
+
+extern "C" int scft_init(const scft_config* cfg) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return init_locked(cfg);
+}
+
+extern "C" int scft_finalize(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (g_one) scft_batch_destroy(g_one);
+  g_one = nullptr;
+  g_one_n = -1;
+  g_init = false;
+  return SCFT_OK;
+}
+
+extern "C" const char* scft_strerror(int code) {
+  switch (code) {
+    case SCFT_OK: return "ok";
+    case SCFT_E_ARG: return "invalid argument";
+    case SCFT_E_HIP: return g_errmsg[0] ? g_errmsg : "HIP runtime error";
+    case SCFT_E_NAN: return "non-finite value in the evaluation";
+    case SCFT_E_PIVOT: return "non-positive tridiagonal pivot";
+    case SCFT_E_NOMEM: return "out of memory";
+    case SCFT_E_STATE: return "library not initialised / no context";
+    case SCFT_E_NOCONV: return "outer solver did not converge";
+    default: return "unknown error";
+  }
+}
+
+extern "C" int scft_last_status(void) { return g_last_status; }
+
+extern "C" int scft_set_target(int n_nodes, const double* phi_t) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (!phi_t) {
+    g_target.clear();
+    return SCFT_OK;
+  }
+  if (n_nodes < 3) return SCFT_E_ARG;
+  g_target.assign(phi_t, phi_t + n_nodes);
+  return SCFT_OK;
+}
+
+// Evaluate r = phi[w] - phi_t for the drop-in entries (w, r on n unknowns).
+static int dropin_eval(int n, const double* w, double* r) {
+  if (!g_init) SCFT_TRY(init_locked(nullptr));
+  const bool dd = (g_cfg.bc == SCFT_BC_DD);
+  const int M = dd ? n + 2 : n + 1;
+  if (n < 1) return SCFT_E_ARG;
+  if (g_one_n != n) {
+    if (g_one) scft_batch_destroy(g_one);
+    g_one = nullptr;
+    g_x.assign(M, 0.0);
+    mesh_uniform(M, g_cfg.L, g_x.data());
+    g_ptu.assign(n, 1.0);
+    const long long xoff = 0;
+    scft_batch_desc d{};
+    d.n_inst = 1;
+    d.M = &M;
+    d.N = &g_cfg.N;
+    d.xoff = &xoff;
+    d.x = g_x.data();
+    d.phi_t = g_ptu.data();
+    d.bc = g_cfg.bc;
+    d.cmat = g_cfg.cmat;
+    d.tune_file = nullptr;
+    SCFT_TRY(scft_batch_create(&g_one, &d));
+    g_one_n = n;
+  }
+  // target: the caller's global (0-based, all nodes), else the installed one,
+  // else the built-in profile
+  g_ptall.assign(M, 1.0);
+  if (f0_given) {
+    std::copy(f0_given, f0_given + M, g_ptall.begin());
+  } else if (static_cast<int>(g_target.size()) == M) {
+    g_ptall = g_target;
+  } else {
+    target_position(M, g_x.data(), g_cfg.tau, dd, g_ptall.data());
+  }
+  for (int u = 0; u < n; u++) g_ptu[u] = g_ptall[u + 1];
+  SCFT_TRY(batch_set_target(g_one, g_ptu.data()));
+  double rn = 0.0;
+  unsigned st = 0;
+  SCFT_TRY(batch_eval_impl(g_one, w, nullptr, r, &rn, &st));
+  if (st & ST_PIVOT) return SCFT_E_PIVOT;
+  if ((st & ST_NAN) || !std::isfinite(rn)) return SCFT_E_NAN;
+  return SCFT_OK;
+}
+
+extern "C" int simple_FEM_1D_transient(int n, double* w, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = dropin_eval(n, w + 1, out + 1);
+  local_iteration++;
+  g_last_status = rc;
+  if (rc != SCFT_OK) funcerr = 1;
+  if (g_cfg.verbose)
+    for (int i = 1; i <= n; i++)
+      printf("in[%d]=%2.15f ; out[%d]=%2.15f ; local_iteration:%d \n", i, w[i], i, out[i],
+             local_iteration);
+  return rc;
+}
+
+static void fixed_point_impl(int n, const double* x, double* xnew) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = dropin_eval(n, x, xnew);
+  g_last_status = rc;
+  local_iteration++;
+  if (rc != SCFT_OK) {
+    funcerr = 1;
+    for (int i = 0; i < n; i++) xnew[i] = NAN;
+    return;
+  }
+  for (int i = 0; i < n; i++) xnew[i] = x[i] + g_cfg.beta * xnew[i];
+}
+
+extern "C" void scft_fixed_point(int n, double* x, double* xnew) { fixed_point_impl(n, x, xnew); }
+
+extern "C" void scft_fixed_point_nr(int n, double* x, double* xnew) {
+  fixed_point_impl(n, x + 1, xnew + 1);
+}
+
+extern "C" void scft_adm_chen_f(int n, double* in, double* out) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  int rc = dropin_eval(n, in, out);
+  g_last_status = rc;
+  local_iteration++;
+  if (rc != SCFT_OK) {
+    funcerr = 1;
+    for (int i = 0; i < n; i++) out[i] = NAN;
+    return;
+  }
+  for (int i = 0; i < n; i++) out[i] = -out[i];
+}

@@ -1,0 +1,524 @@
+// march.hip -- the hot path of the SCFT field evaluation on gfx950, FP64.
+//
+// One workgroup of W wavefronts (P = 64 W lanes) evaluates one instance at a
+// time; lane p owns K consecutive unknowns, so n <= P K.  Per instance:
+//
+//   1. assembly  (1D_FEM.c:84-186, simple_FEM_1D_transient.m:28-79,
+//                 scft.cc:641-651): every lane builds its own rows of
+//                 D = A + dt (B + C) and of the right-hand-side operator A from
+//                 the node coordinates and the field (no atomics: a row reads
+//                 its two adjacent elements itself).
+//   2. factor:   the Thomas pivots u_i = b_i - a_i c_{i-1}/u_{i-1} are a
+//                linear-fractional recurrence; each lane multiplies its K
+//                2x2 matrices [[b_i, -a_i c_{i-1}], [1, 0]], a wave-wide
+//                prefix product (plus a fold over waves) gives every lane the
+//                pivot entering its chunk, then the lane finishes its pivots
+//                sequentially.  The solve then reads
+//                    z_i = r'_i - alpha_i z_{i-1},   x_i = z_i - gamma_i x_{i+1}
+//                with alpha = a/u, gamma = c/u and r' = (A q)/u, all kept in
+//                VGPRs for the whole march.
+//   3. march:    N implicit-Euler steps D q^{j+1} = A q^j (1D_FEM.c:208-228,
+//                step count of DEALII_SCFT/drivescft.cc:130).  Each first-order
+//                recurrence is split into a lane-local pass, a scan of the
+//                lanes' affine maps (6 shuffle levels whose multipliers are
+//                fixed per instance, so only one FMA per level per step), a
+//                fold over waves through LDS, and a lane-local re-run.
+//   4. density:  phi_i = sum_j c_j q_j(i) q_{N-j}(i) (drivescft.cc:184-193 with
+//                romint.c's weights, symmetric in j).  Slabs q_1..q_{N/2-1}
+//                stream to HBM (coalesced: element t of lane p at t*P + p);
+//                from step N/2 on each new slab is paired with the stored
+//                slab N-j, prefetched one step ahead.
+//   5. residual: r = phi - phi_t (1D_FEM.c:274-277), ||r||_inf per instance.
+//
+// No MFMA: nothing here is a dense contraction.
+#include "scft_internal.h"
+
+namespace scft {
+
+template <int W>
+struct MarchShared {
+  double Bw[W];     // forward-scan wave totals (per step)
+  double Cw[W];     // backward-scan wave totals (per step)
+  double Aw[W];     // forward wave transfer products (per instance)
+  double Gw[W];     // backward wave transfer products (per instance)
+  double cx[W];     // last super-diagonal of each wave (setup)
+  double gx[W];     // last gamma of each wave (setup)
+  double Mx[W][4][2];  // wave products of the pivot matrices, double-double (setup)
+  double red[W];    // norm reduction
+  unsigned bad[W];
+};
+
+__device__ __forceinline__ void wg_barrier() { __syncthreads(); }
+
+// value of `v` in lane `l`, as a wave-uniform scalar
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// ---- double-double arithmetic (hi + lo, |lo| <= ulp(hi)/2) for the pivot
+// scan.  The pivots are a projective product of 2x2 matrices whose two
+// eigenvalues nearly coincide when diffusion dominates (dt/h^2 >> 1); in FP64
+// the lane-entry pivots come out ~1e-14 off, and D's conditioning turns that
+// into ~1e-11 per step in q.  Carrying the scan in double-double (once per
+// evaluation) brings the entry pivots back to FP64 accuracy, so the computed
+// LU is as backward-stable as a sequential Thomas factorisation.  Every
+// routine below turns FMA contraction off: fusing a product into a two-sum
+// (s = fma(a, b, c), then s - a*b as another fma) breaks the error-free
+// transformations, and the device compiler does exactly that by default.
+struct dd {
+  double h, l;
+};
+__device__ __forceinline__ dd dd_from(double a) { return {a, 0.0}; }
+__device__ __forceinline__ dd two_sum(double a, double b) {
+#pragma clang fp contract(off)
+  const double s = a + b;
+  const double bb = s - a;
+  const double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ dd quick_two_sum(double a, double b) {
+#pragma clang fp contract(off)
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ dd two_prod(double a, double b) {
+#pragma clang fp contract(off)
+  const double p = a * b;
+  return {p, fma(a, b, -p)};
+}
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+#pragma clang fp contract(off)
+  dd s = two_sum(a.h, b.h);
+  dd t = two_sum(a.l, b.l);
+  s.l += t.h;
+  s = quick_two_sum(s.h, s.l);
+  s.l += t.l;
+  return quick_two_sum(s.h, s.l);
+}
+__device__ __forceinline__ dd dd_mul(dd a, dd b) {
+#pragma clang fp contract(off)
+  dd p = two_prod(a.h, b.h);
+  p.l += a.h * b.l + a.l * b.h;
+  return quick_two_sum(p.h, p.l);
+}
+__device__ __forceinline__ dd dd_scale(dd a, double s) { return {a.h * s, a.l * s}; }  // s = 2^k
+__device__ __forceinline__ double dd_div_to_double(dd a, dd b) {
+#pragma clang fp contract(off)
+  const double q1 = a.h / b.h;
+  const dd r = dd_add(a, dd_mul(b, dd_from(-q1)));
+  return q1 + r.h / b.h;
+}
+
+// 2x2 double-double product R = X * Y, rescaled by a power of two so the
+// largest entry is in [1, 2): the pivot is a ratio of entries, so scaling is
+// harmless and long products neither overflow nor underflow.
+__device__ __forceinline__ void mat_mul_norm(const dd X[4], const dd Y[4], dd R[4]) {
+#pragma clang fp contract(off)
+  const dd r0 = dd_add(dd_mul(X[0], Y[0]), dd_mul(X[1], Y[2]));
+  const dd r1 = dd_add(dd_mul(X[0], Y[1]), dd_mul(X[1], Y[3]));
+  const dd r2 = dd_add(dd_mul(X[2], Y[0]), dd_mul(X[3], Y[2]));
+  const dd r3 = dd_add(dd_mul(X[2], Y[1]), dd_mul(X[3], Y[3]));
+  const double mx = fmax(fmax(fabs(r0.h), fabs(r1.h)), fmax(fabs(r2.h), fabs(r3.h)));
+  const double sc = (mx > 0.0 && isfinite(mx)) ? ldexp(1.0, -ilogb(mx)) : 1.0;
+  R[0] = dd_scale(r0, sc);
+  R[1] = dd_scale(r1, sc);
+  R[2] = dd_scale(r2, sc);
+  R[3] = dd_scale(r3, sc);
+}
+
+__device__ __forceinline__ dd shfl_up_dd(dd v, int s) {
+  return {__shfl_up(v.h, s), __shfl_up(v.l, s)};
+}
+
+// One row of D = A + dt (B + C) and of the right-hand-side operator A for
+// unknown u (node u+1), from its two adjacent elements.  Evaluated without
+// FMA contraction and in the same operation order as the CPU restatement:
+// the rows of D are nearly identical, so a contracted rounding would shift
+// every row the same way and move the diagonal-dominance margin
+// b - |a| - |c| ~ h by eps * dt/h^2 relative per step, which adds up over the
+// march (~1e-10 after a few hundred steps when dt/h^2 ~ 1e3).
+__device__ __forceinline__ void assemble_row(const double* __restrict__ x,
+                                             const double* __restrict__ w, int n, int M, int u,
+                                             double dt, int cmat, double wall0, double wallL,
+                                             double& av, double& bv, double& cv, double& ra,
+                                             double& rc) {
+#pragma clang fp contract(off)
+  if (u < n) {
+    const int i = u + 1;
+    const double xl = x[i - 1], xc = x[i];
+    const bool has_r = (i + 1 < M);
+    const double xr = has_r ? x[i + 1] : xc;
+    const double hl = xc - xl, hr = xr - xc;
+    const double wi = w[u];
+    const double Al = hl / 6.0, Ar = hr / 6.0, Ad = hl / 3.0 + hr / 3.0;
+    const double Bl = -1.0 / hl;
+    const double Br = has_r ? -1.0 / hr : 0.0;
+    const double Bd = 1.0 / hl + (has_r ? 1.0 / hr : 0.0);
+    double Cl, Cd, Cr;
+    if (cmat == 0) {
+      Cl = wi * Al;
+      Cd = wi * Ad;
+      Cr = wi * Ar;
+    } else {
+      const double wl = (u > 0) ? w[u - 1] : wall0;
+      const double wr = (u + 1 < n) ? w[u + 1] : wallL;
+      Cl = hl * (wl + wi) / 12.0;
+      Cd = hl * (wl + 3.0 * wi) / 12.0 + hr * (3.0 * wi + wr) / 12.0;
+      Cr = hr * (wi + wr) / 12.0;
+    }
+    av = Al + dt * (Bl + Cl);
+    bv = Ad + dt * (Bd + Cd);
+    cv = Ar + dt * (Br + Cr);
+    // right-hand side A q split by element: (h_l/6)(q_{i-1} + 2 q_i) +
+    // (h_r/6)(2 q_i + q_{i+1}); wall neighbours are held at q = 0
+    ra = Al;
+    rc = Ar;
+    if (u == 0) av = 0.0;      // left neighbour: the Dirichlet wall
+    if (u == n - 1) cv = 0.0;  // right neighbour: a wall (DD) or none (DN)
+  } else {  // padding: decoupled identity rows, q stays 0
+    av = 0.0;
+    bv = 1.0;
+    cv = 0.0;
+    ra = 0.0;
+    rc = 0.0;
+  }
+}
+
+template <int K, int W>
+__device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0,
+                          double* __restrict__ slab, MarchShared<W>& sh, const int tid) {
+  constexpr int P = 64 * W;
+  const int lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const InstDesc d = a.desc[inst];
+  const int n = __builtin_amdgcn_readfirstlane(d.n);
+  const int N = __builtin_amdgcn_readfirstlane(d.N);
+  const int M = __builtin_amdgcn_readfirstlane(d.M);
+  const double dt = 1.0 / N;
+  const double* __restrict__ x = a.x + d.xo;
+  const double* __restrict__ w = a.w + d.uo;
+  const double* __restrict__ pw = a.pw + d.po;
+  const int cmat = __builtin_amdgcn_readfirstlane(d.cmat);
+  const int u0 = tid * K;
+
+  // ---------------------------------------------------------------- assembly
+  double av[K], bv[K], cv[K], ra[K], rc[K];
+#pragma unroll
+  for (int t = 0; t < K; t++)
+    assemble_row(x, w, n, M, u0 + t, dt, cmat, d.wall0, d.wallL, av[t], bv[t], cv[t], ra[t],
+                 rc[t]);
+
+  // ------------------------------------------------------------ factor scan
+  double cprev = __shfl_up(cv[K - 1], 1);
+  if constexpr (W > 1) {
+    sh.cx[wave] = readlane_d(cv[K - 1], 63);
+    wg_barrier();
+    const double cx_prev = (wave > 0) ? sh.cx[wave - 1] : 0.0;
+    cprev = (lane == 0) ? cx_prev : cprev;
+  } else {
+    cprev = (lane == 0) ? 0.0 : cprev;
+  }
+  dd T[4] = {dd_from(1.0), dd_from(0.0), dd_from(0.0), dd_from(1.0)};
+  {
+    double cp = cprev;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      const dd Mt[4] = {dd_from(bv[t]), two_prod(-av[t], cp), dd_from(1.0), dd_from(0.0)};
+      mat_mul_norm(Mt, T, T);
+      cp = cv[t];
+    }
+  }
+#pragma unroll
+  for (int dd_ = 0; dd_ < 6; dd_++) {
+    const int s = 1 << dd_;
+    dd Tp[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) Tp[e] = shfl_up_dd(T[e], s);
+    if (lane >= s) mat_mul_norm(T, Tp, T);
+  }
+  dd E[4];
+#pragma unroll
+  for (int e = 0; e < 4; e++) E[e] = shfl_up_dd(T[e], 1);
+  dd Pre[4] = {dd_from(1.0), dd_from(0.0), dd_from(0.0), dd_from(1.0)};
+  if constexpr (W > 1) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      sh.Mx[wave][e][0] = readlane_d(T[e].h, 63);
+      sh.Mx[wave][e][1] = readlane_d(T[e].l, 63);
+    }
+    wg_barrier();
+    for (int v = 0; v < wave; v++) {
+      const dd Mv[4] = {{sh.Mx[v][0][0], sh.Mx[v][0][1]}, {sh.Mx[v][1][0], sh.Mx[v][1][1]},
+                        {sh.Mx[v][2][0], sh.Mx[v][2][1]}, {sh.Mx[v][3][0], sh.Mx[v][3][1]}};
+      mat_mul_norm(Mv, Pre, Pre);
+    }
+  }
+  if (lane > 0) {
+    mat_mul_norm(E, Pre, E);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; e++) E[e] = Pre[e];
+  }
+  // pivot of the row just before this lane's chunk (unused for tid 0: a = 0)
+  double up = (tid > 0) ? dd_div_to_double(E[0], E[2]) : 1.0;
+
+  unsigned bad = 0;
+  double al[K], ga[K];
+  {
+#pragma clang fp contract(off)
+    double cp = cprev;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      // Thomas order: l = a / u_prev, u = b - l c_prev
+      const double l = (av[t] != 0.0) ? av[t] / up : 0.0;
+      const double uu = bv[t] - l * cp;
+      if (!(uu > 0.0) || !isfinite(uu)) bad |= ST_PIVOT;
+      const double inv = 1.0 / uu;
+      al[t] = av[t] * inv;
+      ga[t] = cv[t] * inv;
+      ra[t] *= inv;
+      rc[t] *= inv;
+      cp = cv[t];
+      up = uu;
+    }
+  }
+
+  // -------------------------------------------- scan constants of the march
+  double Af = 1.0, Gb = 1.0;
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    Af *= -al[t];
+    Gb *= -ga[t];
+  }
+  double mf[6], mb[6];
+#pragma unroll
+  for (int dd = 0; dd < 6; dd++) {
+    const int s = 1 << dd;
+    const double Ap = __shfl_up(Af, s);
+    const double Gp = __shfl_down(Gb, s);
+    mf[dd] = (lane >= s) ? Af : 0.0;
+    mb[dd] = (lane + s < 64) ? Gb : 0.0;
+    if (lane >= s) Af *= Ap;
+    if (lane + s < 64) Gb *= Gp;
+  }
+  const double Apre_m1 = __shfl_up(Af, 1);   // prod over wave start .. lane-1
+  const double Gsuf_p1 = __shfl_down(Gb, 1);  // prod over lane+1 .. wave end
+  double gl = __shfl_up(ga[K - 1], 1);         // gamma of the row left of me
+  if constexpr (W > 1) {
+    sh.Aw[wave] = readlane_d(Af, 63);
+    sh.gx[wave] = readlane_d(ga[K - 1], 63);
+    sh.Gw[wave] = readlane_d(Gb, 0);
+    wg_barrier();
+    const double gx_prev = (wave > 0) ? sh.gx[wave - 1] : 0.0;
+    gl = (lane == 0) ? gx_prev : gl;
+  } else {
+    gl = (lane == 0) ? 0.0 : gl;
+  }
+
+  // ------------------------------------------------------------------ march
+  double q[K], acc[K], qo[K];
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    q[t] = (u0 + t < n) ? 1.0 : 0.0;
+    acc[t] = 0.0;
+    qo[t] = 0.0;
+  }
+  double ql = (u0 - 1 >= 0 && u0 - 1 < n) ? 1.0 : 0.0;
+  double qr = (u0 + K < n) ? 1.0 : 0.0;
+  const int half = N >> 1;
+
+  for (int j = 1; j <= N; j++) {
+    double rp[K];
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      const double qm = (t == 0) ? ql : q[t - 1];
+      const double qp = (t == K - 1) ? qr : q[t + 1];
+      rp[t] = ra[t] * (qm + 2.0 * q[t]) + rc[t] * (2.0 * q[t] + qp);
+    }
+    // forward recurrence z_i = r'_i - alpha_i z_{i-1}
+    double B = 0.0;
+#pragma unroll
+    for (int t = 0; t < K; t++) B = rp[t] - al[t] * B;
+#pragma unroll
+    for (int dd = 0; dd < 6; dd++) {
+      const double Bp = __shfl_up(B, 1 << dd);
+      B = fma(mf[dd], Bp, B);
+    }
+    double zin = __shfl_up(B, 1);
+    if constexpr (W > 1) {
+      sh.Bw[wave] = readlane_d(B, 63);
+      wg_barrier();
+      double Z = 0.0;
+      for (int v = 0; v < wave; v++) Z = fma(sh.Aw[v], Z, sh.Bw[v]);
+      zin = (lane > 0) ? fma(Apre_m1, Z, zin) : Z;
+    } else {
+      zin = (lane > 0) ? zin : 0.0;
+    }
+    {
+      double z = zin;
+#pragma unroll
+      for (int t = 0; t < K; t++) {
+        z = rp[t] - al[t] * z;
+        rp[t] = z;
+      }
+    }
+    // backward recurrence x_i = z_i - gamma_i x_{i+1}
+    double Cc = 0.0;
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) Cc = rp[t] - ga[t] * Cc;
+#pragma unroll
+    for (int dd = 0; dd < 6; dd++) {
+      const double Cp = __shfl_down(Cc, 1 << dd);
+      Cc = fma(mb[dd], Cp, Cc);
+    }
+    double xin = __shfl_down(Cc, 1);
+    if constexpr (W > 1) {
+      sh.Cw[wave] = readlane_d(Cc, 0);
+      wg_barrier();
+      double X = 0.0;
+      for (int v = W - 1; v > wave; v--) X = fma(sh.Gw[v], X, sh.Cw[v]);
+      xin = (lane < 63) ? fma(Gsuf_p1, X, xin) : X;
+    } else {
+      xin = (lane < 63) ? xin : 0.0;
+    }
+    {
+      double xx = xin;
+#pragma unroll
+      for (int t = K - 1; t >= 0; t--) {
+        xx = rp[t] - ga[t] * xx;
+        q[t] = xx;
+      }
+    }
+    ql = zin - gl * q[0];
+    qr = xin;
+
+    // ------------------------------------------- slabs and the density sum
+    if (j < half) {
+      double* dst = slab + (long long)(j - 1) * (P * K) + tid;
+#pragma unroll
+      for (int t = 0; t < K; t++)
+        if (u0 + t < n) dst[t * P] = q[t];
+    } else {
+      if (j == half) {
+        const double c0 = pw[0];
+#pragma unroll
+        for (int t = 0; t < K; t++) acc[t] = c0 * q[t] * q[t];
+      } else {
+        const double cj = pw[j - half];
+#pragma unroll
+        for (int t = 0; t < K; t++) acc[t] = fma(cj * q[t], qo[t], acc[t]);
+      }
+      // prefetch slab N-j-1 for the next step (slab 0 is the initial ones)
+      const int sn = N - j - 1;
+      if (sn >= 1) {
+        const double* src = slab + (long long)(sn - 1) * (P * K) + tid;
+#pragma unroll
+        for (int t = 0; t < K; t++) qo[t] = (u0 + t < n) ? src[t * P] : 0.0;
+      } else if (sn == 0) {
+#pragma unroll
+        for (int t = 0; t < K; t++) qo[t] = (u0 + t < n) ? 1.0 : 0.0;
+      }
+    }
+  }
+
+  // --------------------------------------------------------------- residual
+  const double* __restrict__ pt = a.phit + d.uo;
+  double* __restrict__ r = a.r + d.uo;
+  double* __restrict__ phi = a.phi ? a.phi + d.uo : nullptr;
+  double mx = 0.0;
+#pragma unroll
+  for (int t = 0; t < K; t++) {
+    const int u = u0 + t;
+    if (u < n) {
+      const double res = acc[t] - pt[u];
+      r[u] = res;
+      if (phi) phi[u] = acc[t];
+      if (!isfinite(res)) bad |= ST_NAN;
+      mx = fmax(mx, fabs(res));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    mx = fmax(mx, __shfl_xor(mx, off));
+    bad |= __shfl_xor(bad, off);
+  }
+  mx = readlane_d(mx, 0);
+  bad = __builtin_amdgcn_readfirstlane(bad);
+  if constexpr (W > 1) {
+    sh.red[wave] = mx;
+    sh.bad[wave] = bad;
+    wg_barrier();
+    for (int v = 0; v < W; v++) {
+      mx = fmax(mx, sh.red[v]);
+      bad |= sh.bad[v];
+    }
+    wg_barrier();  // sh.red / sh.bad are reused by the next instance
+  }
+  // every lane of wave 0 stores the same (uniform) values: no lane branch
+  if (wave == 0) {
+    a.rnorm[inst] = (bad & ST_NAN) ? __builtin_nan("") : mx;
+    a.status[inst] = st0 | bad;
+  }
+}
+
+template <int K, int W>
+__global__ void __launch_bounds__(64 * W) march_kernel(MarchArgs a) {
+  __shared__ MarchShared<W> sh;
+  const int tid = threadIdx.x;
+  double* slab = a.slabs + (long long)blockIdx.x * a.slab_stride;
+  for (int it = blockIdx.x; it < a.n_items; it += gridDim.x) {
+    const int inst = __builtin_amdgcn_readfirstlane(a.items[it]);
+    const unsigned st = __builtin_amdgcn_readfirstlane(a.status[inst]);
+    if (st & ST_DONE_MASK) continue;
+    march_one<K, W>(a, inst, st, slab, sh, tid);
+  }
+}
+
+// ------------------------------------------------------------------ dispatch
+// (K, W) variants: K * 8 VGPRs of coefficients per lane must leave room for
+// the register file share of a 64W-lane workgroup (no spills).
+#define SCFT_MARCH_VARIANTS(X) \
+  X(1, 1) X(2, 1) X(4, 1) X(8, 1) X(16, 1) \
+  X(1, 2) X(2, 2) X(4, 2) X(8, 2) X(16, 2) \
+  X(1, 4) X(2, 4) X(4, 4) X(8, 4) X(16, 4) \
+  X(1, 8) X(2, 8) X(4, 8) X(8, 8)          \
+  X(1, 16) X(2, 16)
+
+static const void* march_fn(int K, int W) {
+#define X(k, w) \
+  if (K == k && W == w) return reinterpret_cast<const void*>(&march_kernel<k, w>);
+  SCFT_MARCH_VARIANTS(X)
+#undef X
+  return nullptr;
+}
+
+bool march_variant_exists(int K, int W) { return march_fn(K, W) != nullptr; }
+
+hipError_t launch_march(int K, int W, const MarchArgs& a, int grid, hipStream_t s) {
+#define X(k, w)                                                                 \
+  if (K == k && W == w) {                                                       \
+    hipLaunchKernelGGL((march_kernel<k, w>), dim3(grid), dim3(64 * w), 0, s, a); \
+    return hipGetLastError();                                                   \
+  }
+  SCFT_MARCH_VARIANTS(X)
+#undef X
+  return hipErrorInvalidValue;
+}
+
+int march_max_grid(int K, int W) {
+  const void* f = march_fn(K, W);
+  if (!f) return 0;
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 64 * W, 0) != hipSuccess) return 0;
+  if (per < 1) per = 1;
+  return per * cus;
+}
+
+}  // namespace scft

@@ -1,0 +1,88 @@
+// scft_internal.h -- device data layout shared by the kernels and the host
+// side of libscft_hip.so (not part of the public C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace scft {
+
+// One SCFT instance as the kernels see it.  Unknown u <-> mesh node u+1.
+struct InstDesc {
+  int n;          // unknowns (M-2 for DD, M-1 for DN)
+  int N;          // implicit-Euler contour steps, 2^k >= 16
+  int M;          // mesh nodes
+  int bc;         // SCFT_BC_DD / SCFT_BC_DN
+  int cmat;       // SCFT_C_ROWSCALED / SCFT_C_CONSISTENT
+  int pad;
+  long long xo;   // offset of the M node coordinates in `x`
+  long long uo;   // offset of the n unknowns in w / phi_t / phi / r
+  long long po;   // offset of the N/2+1 pairing weights in `pw`
+  long long ho;   // offset of the history ring (S*n doubles) of the mixer
+  double wall0;   // field at node 0 (only the consistent C reads it)
+  double wallL;   // field at node M-1 for DD (consistent C only)
+};
+
+// Arguments of one persistent march launch (one size bucket).
+struct MarchArgs {
+  const InstDesc* desc;
+  const double* x;        // node coordinates
+  const double* w;        // field on unknowns
+  const double* phit;     // target on unknowns
+  const double* pw;       // Romberg pairing weights
+  double* phi;            // out: density on unknowns (nullable)
+  double* r;              // out: residual phi - phi_t on unknowns
+  double* rnorm;          // out: ||r||_inf per instance
+  unsigned* status;       // per-instance status word
+  double* slabs;          // q-slab scratch, one region per workgroup
+  long long slab_stride;  // doubles per workgroup region
+  const int* items;       // instance ids of this bucket, largest first
+  int n_items;
+};
+
+constexpr unsigned ST_CONVERGED = 1u;
+constexpr unsigned ST_NAN = 2u;
+constexpr unsigned ST_PIVOT = 4u;
+constexpr unsigned ST_MAXITER = 8u;
+constexpr unsigned ST_FALLBACK = 16u;
+constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER;
+
+struct MixState {
+  int k;         // evaluations so far
+  int head;      // next ring slot
+  int stored;    // valid ring entries
+  int restarts;
+  double best;   // best ||r||_inf so far
+};
+
+struct AndersonArgs {
+  const InstDesc* desc;
+  double* w;            // field on unknowns (updated in place)
+  const double* r;      // plain residual phi - phi_t from the march
+  const double* rnorm;  // ||r||_inf from the march
+  const double* phit;   // target on unknowns
+  const double* wcold;  // cold-start fields (nullable)
+  double* hw;           // history of fields   (S * n per instance at ho)
+  double* hr;           // history of residuals
+  MixState* st;
+  unsigned* status;
+  int* iters;           // evaluations used (written at convergence / stop)
+  int* active;          // count of instances still running after this sweep
+  int n_inst;
+  int history;          // m (ring of m+1 slots)
+  int transform;        // 0 plain, 1 relative, 2 log
+  int max_iter;
+  double beta, tol, step_cap, restart_factor, reg, floor;
+  double fallback_thresh;  // absolute threshold (<= 0: off), first sweep only
+};
+
+// Launch one bucket.  Returns hipSuccess or the launch error.
+hipError_t launch_march(int K, int W, const MarchArgs& a, int grid,
+                        hipStream_t s);
+// Largest grid of co-resident workgroups for a variant (persistent launch).
+int march_max_grid(int K, int W);
+bool march_variant_exists(int K, int W);
+hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
+hipError_t launch_reset(int* counters, int n, hipStream_t s);
+constexpr int MIX_MAX_HISTORY = 12;
+
+}  // namespace scft

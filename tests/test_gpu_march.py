@@ -1,0 +1,180 @@
+"""Device parity: the HIP march / density / residual against the CPU oracle.
+
+Bar (FP64 throughout):
+  * propagator, per stored step j:  ||q_dev,j - q_cpu,j||_inf <= 1e-10 ||q_cpu,j||_inf
+  * density and residual:           |phi_dev - phi_cpu| <= 1e-10 (phi is O(1))
+Cases cover both boundary conditions, both w-mass variants, uniform and graded
+meshes, the lane/wave padding edges around 64 K W, and fields from the
+reference's own fixtures.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import fixtures, instance, unknowns
+
+pytestmark = pytest.mark.gpu
+
+STEP_RTOL = 1e-10
+PHI_ATOL = 1e-10
+
+
+def _scft():
+    import scft_amd
+    return scft_amd
+
+
+def _cases():
+    fx = fixtures()
+    eta32 = np.array(fx["res_m32"]["eta"])
+    eta1024 = np.array(fx["res_m1024"]["eta"])
+    rng = np.random.default_rng(7)
+    c = []
+    c.append(("ref_m32_DD_res_eta", 32, 2048, O.L_REF, O.BC_DD, O.C_ROWSCALED, None, eta32))
+    c.append(("ref_m1024_DD_res_eta", 1024, 2048, O.L_REF, O.BC_DD, O.C_ROWSCALED, None, eta1024))
+    c.append(("cfg1_m32_DN", 32, 2048, 3.72, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("cfg2_m1024_DN", 1024, 2048, 3.72, O.BC_DN, O.C_ROWSCALED, None, None))
+    for m in (2, 3, 63, 64, 65, 66, 129, 255, 257, 511, 513, 1023, 1025):
+        bc = O.BC_DD if m % 2 else O.BC_DN
+        c.append((f"edge_m{m}", m, 256, 2.0 + m % 7, bc, O.C_ROWSCALED, None, None))
+    c.append(("big_m2047_DN", 2047, 512, 8.0, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("big_m2049_DD", 2049, 512, 8.0, O.BC_DD, O.C_ROWSCALED, None, None))
+    c.append(("max_m4096_DN", 4096, 512, 6.0, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("max_m4097_DD", 4097, 256, 6.0, O.BC_DD, O.C_ROWSCALED, None, None))
+    c.append(("consistent_m100_DD", 100, 256, 3.0, O.BC_DD, O.C_CONSISTENT, None, None))
+    c.append(("consistent_m1000_DN", 1000, 512, 4.0, O.BC_DN, O.C_CONSISTENT, None, None))
+    c.append(("graded_DN", None, 1024, 5.0, O.BC_DN, O.C_ROWSCALED, (1e-3, 0.2), None))
+    c.append(("rough_w_m300", 300, 1024, 3.5, O.BC_DN, O.C_ROWSCALED, None,
+              rng.uniform(-5.0, 20.0, 301)))
+    c.append(("steps_4096", 200, 4096, 3.0, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("steps_16", 40, 16, 3.0, O.BC_DD, O.C_ROWSCALED, None, None))
+    return c
+
+
+CASES = _cases()
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c[0] for c in CASES])
+def test_march_matches_oracle(case):
+    S = _scft()
+    name, m, N, L, bc, cmat, graded, w = case
+    if graded is not None:
+        h0, ell = graded
+        m = O.graded_elements(L, h0, ell, 63, 4095)
+        inst = instance(m, L, N, bc=bc, graded=ell, seed=11)
+    else:
+        inst = instance(m, L, N, bc=bc, seed=3, w=w)
+    x, pt, wf = inst["x"], inst["pt"], inst["w"]
+    if bc == O.BC_DD:
+        wf = wf.copy()
+        wf[0] = wf[-1] = 0.0
+    else:
+        wf = wf.copy()
+        wf[0] = 0.0
+    print(f"case {name}: M={len(x)} N={N} bc={bc} cmat={cmat}", flush=True)
+    ref = O.evaluate(x, wf, pt, N, bc=bc, cmat=cmat, want_q=True)
+    assert ref["rc"] == 0
+    b = S.Batch([x], [N], [unknowns(pt, bc)], bc=bc, cmat=cmat)
+    out = b.eval([unknowns(wf, bc)])
+    assert out["status"][0] == 0, out["status"]
+    # per-step propagator
+    if N >= 6:
+        slabs = b.debug_slabs(0)
+        qc = ref["q"][1:N // 2]
+        qc = qc[:, 1:-1] if bc == O.BC_DD else qc[:, 1:]
+        err = np.abs(slabs - qc).max(axis=1) / np.abs(qc).max(axis=1)
+        assert err.max() <= STEP_RTOL, f"step {err.argmax() + 1}: {err.max():.3e}"
+    phi_ref = unknowns(ref["phi"], bc)
+    np.testing.assert_allclose(out["phi"][0], phi_ref, rtol=0, atol=PHI_ATOL)
+    np.testing.assert_allclose(out["r"][0], unknowns(ref["r"], bc), rtol=0, atol=PHI_ATOL)
+    assert abs(out["rnorm"][0] - ref["rnorm"]) <= PHI_ATOL
+    b.close()
+
+
+def test_mixed_batch_matches_oracle():
+    """Uneven sizes in one launch per bucket (every bucket populated)."""
+    S = _scft()
+    rng = np.random.default_rng(20261202)
+    ms = [32, 33, 64, 100, 127, 128, 200, 255, 256, 300, 511, 512, 700, 1023, 1024, 1500,
+          2047, 2048, 3000, 4096]
+    ms += list(np.exp(rng.uniform(np.log(32), np.log(4096), 12)).round().astype(int))
+    Ns = [int(rng.choice([512, 1024, 2048])) for _ in ms]
+    Ls = rng.uniform(2.0, 8.0, len(ms))
+    insts = [instance(int(m), float(L), N, bc=O.BC_DN, seed=100 + i)
+             for i, (m, L, N) in enumerate(zip(ms, Ls, Ns))]
+    for it in insts:
+        it["w"][0] = 0.0
+    ref = O.batch_eval([it["M"] for it in insts], Ns, [it["x"] for it in insts],
+                       [it["pt"] for it in insts], [it["w"] for it in insts], bc=O.BC_DN)
+    assert ref["rc"] == 0
+    b = S.Batch([it["x"] for it in insts], Ns, [it["pt"][1:] for it in insts], bc=O.BC_DN)
+    out = b.eval([it["w"][1:] for it in insts])
+    assert (out["status"] == 0).all()
+    for k in range(len(insts)):
+        np.testing.assert_allclose(out["phi"][k], ref["phi"][k][1:], rtol=0, atol=PHI_ATOL,
+                                   err_msg=f"instance {k} m={ms[k]} N={Ns[k]}")
+    np.testing.assert_allclose(out["rnorm"], ref["rnorm"], rtol=0, atol=PHI_ATOL)
+    # evaluating twice gives identical bits (no state leaks between sweeps)
+    out2 = b.eval([it["w"][1:] for it in insts])
+    for k in range(len(insts)):
+        assert np.array_equal(out["phi"][k], out2["phi"][k])
+    b.close()
+
+
+def test_full_size_properties():
+    """Largest configuration-3 shapes (n = 4095/2048 unknowns, N = 4096) checked
+    through size-independent properties instead of the CPU march:
+      * mirror symmetry: DD with a symmetric field gives phi(x) = phi(L - x);
+      * half-domain equivalence: DN on [0, L/2] with the left half of that
+        field reproduces the DD density on the left half exactly (the natural
+        row at x = L/2 is half the symmetric DD row, for A, B and C alike)."""
+    S = _scft()
+    L, m, N = 8.0, 4096, 4096
+    x = O.mesh_uniform(m + 1, L)
+    w = 2.0 * np.cos(2 * np.pi * x / L) + 0.5 * np.exp(-((x - L / 2) ** 2))
+    w = 0.5 * (w + w[::-1])
+    pt = O.target_position(x, O.TAU, True)
+    bd = S.Batch([x], [N], [pt[1:-1]], bc=S.BC_DD)
+    phi_dd = bd.eval([w[1:-1]])["phi"][0]
+    # FP64 floor of this shape: the oracle's own sequential sweep is asymmetric
+    # by 4e-12 and differs from its extended-precision march by 4e-12
+    assert np.abs(phi_dd - phi_dd[::-1]).max() <= 2e-11 * np.abs(phi_dd).max()
+    xh = x[: m // 2 + 1].copy()
+    bn = S.Batch([xh], [N], [pt[1 : m // 2 + 1]], bc=S.BC_DN)
+    phi_dn = bn.eval([w[1 : m // 2 + 1]])["phi"][0]
+    np.testing.assert_allclose(phi_dn, phi_dd[: m // 2], rtol=0, atol=2e-11)
+    bd.close()
+    bn.close()
+
+
+def test_dropin_evaluator_matches_oracle():
+    """simple_FEM_1D_transient with NR 1-based arrays (1D_FEM.c:47-48)."""
+    S = _scft()
+    S.init(N=2048, L=O.L_REF, bc=S.BC_DD)
+    fx = fixtures()
+    eta = np.array(fx["res_m32"]["eta"])
+    n = 31
+    w_nr = np.zeros(n + 1)
+    w_nr[1:] = eta[1:-1]
+    out_nr = np.zeros(n + 1)
+    before = S.local_iteration()
+    rc = S.simple_FEM_1D_transient(w_nr, out_nr)
+    assert rc == 0 and S.funcerr() == 0
+    assert S.local_iteration() == before + 1
+    x = O.mesh_uniform(33, O.L_REF)
+    pt = O.target_position(x, O.TAU, True)
+    wf = np.zeros(33)
+    wf[1:-1] = eta[1:-1]
+    ref = O.evaluate(x, wf, pt, 2048, bc=O.BC_DD)
+    np.testing.assert_allclose(out_nr[1:], ref["r"][1:-1], rtol=0, atol=PHI_ATOL)
+    # the .res field is Wang's solution of a different discretisation: its
+    # IE-FEM residual is O(1e-2), not a solution of this one
+    assert 1e-3 < np.abs(out_nr[1:]).max() < 0.2
+    # fixed-point and adm_chen forms share the evaluation
+    xnew = np.zeros(n)
+    S.lib.scft_fixed_point(n, np.ascontiguousarray(w_nr[1:]), xnew)
+    np.testing.assert_allclose(xnew, w_nr[1:] + 0.1 * ref["r"][1:-1], rtol=0, atol=1e-10)
+    o2 = np.zeros(n)
+    S.lib.scft_adm_chen_f(n, np.ascontiguousarray(w_nr[1:]), o2)
+    np.testing.assert_allclose(o2, -ref["r"][1:-1], rtol=0, atol=PHI_ATOL)
+    S.finalize()
