@@ -149,6 +149,7 @@ typedef struct scft_anderson_opts {
   double fallback_factor; /* warm starts whose first ||r|| exceed
                              fallback_factor * median restart cold (<=0: off) */
   int check_every;        /* host polls the active count every k sweeps */
+  double progress_seconds;/* > 0: print a progress line to stderr this often */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);
@@ -164,15 +165,31 @@ typedef struct scft_solve_stats {
 
 /* w: in = initial fields, out = solutions (host, on unknowns); w_cold
  * (nullable) = cold-start fields used by the fallback; iters[k] / status[k]
- * (nullable) per instance. */
+ * (nullable) per instance.  Equivalent to upload + solve_resident + download;
+ * stats->seconds covers the device-resident part only.  Returns SCFT_OK when
+ * every instance converged, SCFT_E_NOCONV otherwise. */
 int scft_batch_solve(scft_batch *b, const scft_anderson_opts *o, double *w,
                      const double *w_cold, int *iters, unsigned *status,
                      scft_solve_stats *stats);
+/* The three phases separately: host fields into HBM; the solve with inputs
+ * and outputs resident in HBM (graph-launched sweeps, the host polls a
+ * 4-byte active count every check_every sweeps); results back to the host. */
+int scft_batch_upload(scft_batch *b, const double *w, const double *w_cold);
+int scft_batch_solve_resident(scft_batch *b, const scft_anderson_opts *o,
+                              scft_solve_stats *stats);
+int scft_batch_download(scft_batch *b, double *w, int *iters, unsigned *status,
+                        scft_solve_stats *stats);
 
 /* Timing helpers used by bench.py: time `reps` evaluations of the current
  * device field with HIP events on the launch stream. */
 int scft_batch_time_eval(scft_batch *b, const double *w, int reps,
                          double *ms_per_eval, double *march_ms);
+/* Per size bucket (largest first): average duration of one persistent march
+ * launch over `reps` (HIP events on the launch stream), its algorithmic HBM
+ * bytes (q-slabs written and read back plus O(n) I/O) and instance count. */
+int scft_batch_time_buckets(scft_batch *b, const double *w, int reps,
+                            int max_buckets, int *n_buckets, double *ms,
+                            double *bytes, int *n_items);
 
 /* Autotuner: time every (nodes-per-lane, waves) variant that fits each size
  * bucket and keep the fastest; written to desc.tune_file when given. */

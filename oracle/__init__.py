@@ -118,6 +118,10 @@ def lib():
                                       _ip, _ip, _dp]
         L.orc_batch_eval.argtypes = [C.c_int, _ip, _ip, _lp, _dp, _dp, _dp, C.c_int, C.c_int,
                                      C.c_int, _dp, _dp, _dp]
+        L.orc_global_setup.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_long]
+        L.orc_global_calls.restype = C.c_long
+        L.orc_ctx_init.argtypes = [C.c_void_p, C.POINTER(OrcProblem), C.c_double]
+        L.orc_ctx_free.argtypes = [C.c_void_p]
         L.orc_mlp_param_count.restype = C.c_long
         L.orc_mlp_param_count.argtypes = [C.c_int, _ip]
         L.orc_mlp_forward.argtypes = [C.c_int, _ip, _dp, C.c_int, _dp, _dp]
@@ -339,6 +343,38 @@ def batch_eval(Ms, Ns, xs, phits, ws, bc=BC_DN, cmat=C_ROWSCALED, threads=0):
     rc = lib().orc_batch_eval(len(Ms), Ms, Ns, off, x, pt, w, bc, cmat, threads, phi, r, rn)
     return dict(rc=rc, phi=[phi[o:o + m] for o, m in zip(off, Ms)],
                 r=[r[o:o + m] for o, m in zip(off, Ms)], rnorm=rn)
+
+
+class EvalCtx:
+    """An oracle evaluation context (orc_eval_ctx) for a fixed problem."""
+
+    SIZE = 256  # bytes, larger than sizeof(orc_eval_ctx)
+
+    def __init__(self, x, phi_t, N, bc=BC_DD, cmat=C_ROWSCALED, beta=0.1):
+        self.p = make_problem(x, phi_t, N, bc, cmat)
+        self.buf = C.create_string_buffer(self.SIZE)
+        rc = lib().orc_ctx_init(self.buf, C.byref(self.p), beta)
+        if rc != 0:
+            raise RuntimeError("orc_ctx_init failed")
+
+    @property
+    def ptr(self):
+        return C.cast(self.buf, C.c_void_p)
+
+    def close(self):
+        if self.buf is not None:
+            lib().orc_ctx_free(self.buf)
+            self.buf = None
+
+
+def global_callbacks(ctx, n, trace_len=0, limit=0):
+    """Install ctx behind the reference-signature callbacks; returns
+    (fixed_point_fn_ptr, vecfunc_nr_fn_ptr, trace_array)."""
+    trace = np.zeros(max(1, trace_len * n))
+    lib().orc_global_setup(ctx.ptr, trace.ctypes.data, len(trace), limit)
+    fp = C.cast(lib().orc_global_fixed_point, C.c_void_p)
+    vf = C.cast(lib().orc_global_vecfunc_nr, C.c_void_p)
+    return fp, vf, trace
 
 
 # ---------------------------------------------------------------- MLP

@@ -123,3 +123,58 @@ int orc_batch_eval(int n_inst, const int *M, const int *N, const long *xoff,
   }
   return bad ? ORC_E_NAN : ORC_OK;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Callbacks with the reference's exact signatures (no context argument), so */
+/* the reference's compiled adm/broydn can drive the oracle evaluator:       */
+/*   adm.c:25-27    void funcvmix(int n, double *x, double *xnew)  (flag 0)  */
+/*   broydn.c:45-46 void vecfunc(int n, double v[], double f[])   (1-based)  */
+/* Every call's input is appended to a trace for trajectory comparisons;     */
+/* after `limit` calls the fixed-point form returns x itself, which ends     */
+/* adm (its tolerance is hard-coded, adm.c:29).                              */
+/* ------------------------------------------------------------------------ */
+static orc_eval_ctx *g_ctx = NULL;
+static double *g_trace = NULL;
+static long g_trace_cap = 0, g_calls = 0, g_limit = 0;
+
+void orc_global_setup(orc_eval_ctx *c, double *trace, long trace_cap, long limit) {
+  g_ctx = c;
+  g_trace = trace;
+  g_trace_cap = trace_cap;
+  g_calls = 0;
+  g_limit = limit;
+}
+
+long orc_global_calls(void) { return g_calls; }
+
+static void record(int n, const double *x) {
+  if (g_trace && (g_calls + 1) * n <= g_trace_cap)
+    memcpy(g_trace + g_calls * n, x, sizeof(double) * n);
+  g_calls++;
+}
+
+void orc_global_fixed_point(int n, double *x, double *xnew) {
+  record(n, x);
+  if (g_limit > 0 && g_calls > g_limit) {
+    memcpy(xnew, x, sizeof(double) * n);
+    return;
+  }
+  orc_fixed_point_cb(n, x, xnew, g_ctx);
+}
+
+void orc_global_vecfunc_nr(int n, double *v, double *f) {
+  record(n, v + 1);
+  orc_residual_cb(n, v + 1, f + 1, g_ctx);
+}
+
+/* The traced callbacks in the oracle's own calling conventions. */
+void orc_global_fixed_point_ctx(int n, const double *x, double *xnew, void *ctx) {
+  (void)ctx;
+  orc_global_fixed_point(n, (double *)x, xnew);
+}
+
+int orc_global_residual_ctx(int n, const double *x, double *f, void *ctx) {
+  (void)ctx;
+  record(n, x);
+  return orc_residual_cb(n, x, f, g_ctx);
+}

@@ -40,6 +40,13 @@ int orc_batch_eval(int n_inst, const int *M, const int *N, const long *xoff,
                    int bc, int cmat, int threads, double *phi, double *r,
                    double *rnorm);
 
+void orc_global_setup(orc_eval_ctx *c, double *trace, long trace_cap, long limit);
+long orc_global_calls(void);
+void orc_global_fixed_point(int n, double *x, double *xnew);
+void orc_global_vecfunc_nr(int n, double *v, double *f);
+void orc_global_fixed_point_ctx(int n, const double *x, double *xnew, void *ctx);
+int orc_global_residual_ctx(int n, const double *x, double *f, void *ctx);
+
 #ifdef __cplusplus
 }
 #endif

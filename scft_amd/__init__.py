@@ -59,7 +59,8 @@ class AndersonOpts(C.Structure):
                 ("max_iter", C.c_int), ("step_cap", C.c_double),
                 ("restart_factor", C.c_double), ("regularization", C.c_double),
                 ("transform", C.c_int), ("floor", C.c_double),
-                ("fallback_factor", C.c_double), ("check_every", C.c_int)]
+                ("fallback_factor", C.c_double), ("check_every", C.c_int),
+                ("progress_seconds", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -93,8 +94,15 @@ def _load():
     L.scft_default_anderson.restype = None
     L.scft_batch_solve.argtypes = [C.c_void_p, C.POINTER(AndersonOpts), _dp, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.POINTER(SolveStats)]
+    L.scft_batch_upload.argtypes = [C.c_void_p, _dp, C.c_void_p]
+    L.scft_batch_solve_resident.argtypes = [C.c_void_p, C.POINTER(AndersonOpts),
+                                            C.POINTER(SolveStats)]
+    L.scft_batch_download.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                      C.POINTER(SolveStats)]
     L.scft_batch_time_eval.argtypes = [C.c_void_p, _dp, C.c_int, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double)]
+    L.scft_batch_time_buckets.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int,
+                                          C.POINTER(C.c_int), _dp, _dp, _ip]
     L.scft_batch_autotune.argtypes = [C.c_void_p, C.c_int]
     L.scft_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), _ip, _ip, _ip, C.c_int]
     L.scft_debug_slabs.argtypes = [C.c_void_p, C.c_int, _dp]
@@ -153,9 +161,9 @@ def local_iteration() -> int:
 # ------------------------------------------------------------------ batches
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
-                  fallback_factor=0.0, check_every=8):
+                  fallback_factor=0.0, check_every=8, progress_seconds=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
-                        transform, floor, fallback_factor, check_every)
+                        transform, floor, fallback_factor, check_every, progress_seconds)
 
 
 class Batch:
@@ -231,6 +239,31 @@ class Batch:
                     converged=stats.converged, seconds=stats.seconds,
                     node_steps=stats.node_steps, bytes=stats.bytes, rc=rc)
 
+    def upload(self, w_parts, w_cold_parts=None):
+        w = self.join(w_parts)
+        wc = self.join(w_cold_parts) if w_cold_parts is not None else None
+        self._up = (w, wc)
+        check(lib.scft_batch_upload(self.h, w, wc.ctypes.data if wc is not None else None),
+              "scft_batch_upload")
+
+    def solve_resident(self, opts):
+        st = SolveStats()
+        check(lib.scft_batch_solve_resident(self.h, C.byref(opts), C.byref(st)),
+              "scft_batch_solve_resident")
+        return dict(sweeps=st.sweeps, seconds=st.seconds)
+
+    def download(self):
+        w = np.empty(self.total)
+        it = np.zeros(len(self.n), dtype=np.int32)
+        stv = np.zeros(len(self.n), dtype=np.uint32)
+        st = SolveStats()
+        rc = lib.scft_batch_download(self.h, w.ctypes.data, it.ctypes.data, stv.ctypes.data,
+                                     C.byref(st))
+        if rc not in (0, 7):
+            check(rc, "scft_batch_download")
+        return dict(w=self.split(w), iters=it, status=stv, converged=st.converged,
+                    node_steps=st.node_steps, bytes=st.bytes, sweeps=st.sweeps)
+
     def time_eval(self, w_parts, reps=5):
         w = self.join(w_parts)
         ms = C.c_double()
@@ -238,6 +271,16 @@ class Batch:
         check(lib.scft_batch_time_eval(self.h, w, reps, C.byref(ms), C.byref(mm)),
               "scft_batch_time_eval")
         return ms.value
+
+    def time_buckets(self, w_parts, reps=3):
+        """[(n_items, ms per launch, algorithmic bytes per launch)] per bucket."""
+        w = self.join(w_parts)
+        nb = C.c_int()
+        ms, by = np.zeros(16), np.zeros(16)
+        ni = np.zeros(16, dtype=np.int32)
+        check(lib.scft_batch_time_buckets(self.h, w, reps, 16, C.byref(nb), ms, by, ni),
+              "scft_batch_time_buckets")
+        return [(int(ni[i]), float(ms[i]), float(by[i])) for i in range(nb.value)]
 
     def autotune(self, reps=2):
         check(lib.scft_batch_autotune(self.h, reps), "scft_batch_autotune")

@@ -63,6 +63,8 @@ struct Bucket {
   int* d_items = nullptr;
   int grid = 0;
   long long slab_stride = 0;  // doubles per workgroup
+  long long slab_base = 0;    // this bucket's region in the scratch (doubles)
+  int stream = 0;             // which of the batch's streams launches it
 };
 
 int default_K(int cap) { return std::min(cap / 64, 16); }
@@ -84,7 +86,11 @@ struct scft_batch {
   long long total_u = 0, total_x = 0, total_pw = 0, total_h = 0;
   std::string tune_file;
   std::vector<Bucket> buckets;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // main stream: copies, resets, the mixer
+  // buckets run concurrently on the main stream plus kSide side streams
+  static constexpr int kSide = 3;
+  hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
+  hipEvent_t fork = nullptr, join[kSide] = {nullptr, nullptr, nullptr};
   // device state
   InstDesc* d_desc = nullptr;
   double *d_x = nullptr, *d_w = nullptr, *d_phit = nullptr, *d_pw = nullptr;
@@ -100,6 +106,10 @@ struct scft_batch {
   int* d_iters = nullptr;
   // algorithmic work per full sweep (all instances)
   double node_steps_all = 0.0, bytes_all = 0.0;
+  bool has_wcold = false;
+  std::vector<MixState> ms0;
+  int* h_active = nullptr;  // pinned
+  int last_sweeps = 0;
 };
 
 // Algorithmic HBM bytes of one evaluation: q-slabs 1..N/2-1 written once and
@@ -108,9 +118,14 @@ static double eval_bytes(int n, int N) {
   return 8.0 * n * (2.0 * (N / 2 - 1)) + 40.0 * n;
 }
 
+// Every non-empty bucket owns a scratch region (the buckets of one sweep run
+// concurrently): grid_b workgroups x (N_max/2 - 1) slabs x 64 W K doubles.
+// Buckets are spread over the main and side streams, largest work first.
 static int size_scratch(scft_batch* b) {
   long long need = 0;
-  for (auto& bk : b->buckets) {
+  std::vector<std::pair<double, int>> order;
+  for (size_t i = 0; i < b->buckets.size(); i++) {
+    Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
     if (!march_variant_exists(bk.K, bk.W)) {
       snprintf(g_errmsg, sizeof g_errmsg, "no march variant K=%d W=%d", bk.K, bk.W);
@@ -120,7 +135,21 @@ static int size_scratch(scft_batch* b) {
     const int mg = march_max_grid(bk.K, bk.W);
     bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), mg));
     bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2 - 1)) * P * bk.K;
-    need = std::max(need, bk.slab_stride * bk.grid);
+    bk.slab_base = need;
+    need += bk.slab_stride * bk.grid;
+    double work = 0.0;
+    for (int k : bk.items) work += static_cast<double>(b->desc[k].n) * b->desc[k].N;
+    order.push_back({work, static_cast<int>(i)});
+  }
+  std::sort(order.begin(), order.end(), std::greater<std::pair<double, int>>());
+  // greedy: next bucket onto the least loaded stream
+  double load[1 + scft_batch::kSide] = {0.0, 0.0, 0.0, 0.0};
+  for (auto& o : order) {
+    int s = 0;
+    for (int t = 1; t <= scft_batch::kSide; t++)
+      if (load[t] < load[s]) s = t;
+    b->buckets[o.second].stream = s;
+    load[s] += o.first;
   }
   if (need > b->slab_doubles) {
     dfree(b->d_slabs);
@@ -161,7 +190,7 @@ static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_id
   a.r = b->d_r;
   a.rnorm = b->d_rnorm;
   a.status = b->d_status;
-  a.slabs = b->d_slabs;
+  a.slabs = b->d_slabs + bk.slab_base;
   a.slab_stride = bk.slab_stride;
   a.items = bk.d_items;
   a.n_items = static_cast<int>(bk.items.size());
@@ -169,16 +198,31 @@ static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_id
   return a;
 }
 
-// One evaluation sweep of every instance not marked done: counters reset,
-// then one persistent launch per non-empty bucket, largest bucket first.
+// One evaluation sweep of every instance not marked done: counters reset on
+// the main stream, then one persistent launch per non-empty bucket, the
+// buckets spread over the main and side streams (fork/join through events,
+// which stream capture turns into parallel graph branches).
+static hipStream_t stream_of(scft_batch* b, int s) { return s == 0 ? b->stream : b->side[s - 1]; }
+
 static int enqueue_sweep(scft_batch* b) {
   const int nb = static_cast<int>(b->buckets.size());
   HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+  bool used[1 + scft_batch::kSide] = {true, false, false, false};
+  for (auto& bk : b->buckets)
+    if (!bk.items.empty()) used[bk.stream] = true;
+  HIPCHK(hipEventRecord(b->fork, b->stream));
+  for (int s = 1; s <= scft_batch::kSide; s++)
+    if (used[s]) HIPCHK(hipStreamWaitEvent(b->side[s - 1], b->fork, 0));
   for (int i = nb - 1; i >= 0; i--) {
     const Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
-    HIPCHK(launch_march(bk.K, bk.W, march_args(b, bk, i), bk.grid, b->stream));
+    HIPCHK(launch_march(bk.K, bk.W, march_args(b, bk, i), bk.grid, stream_of(b, bk.stream)));
   }
+  for (int s = 1; s <= scft_batch::kSide; s++)
+    if (used[s]) {
+      HIPCHK(hipEventRecord(b->join[s - 1], b->side[s - 1]));
+      HIPCHK(hipStreamWaitEvent(b->stream, b->join[s - 1], 0));
+    }
   return SCFT_OK;
 }
 
@@ -278,6 +322,12 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
   };
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return fail(SCFT_E_HIP);
+  for (int s = 0; s < scft_batch::kSide; s++)
+    if (hipStreamCreateWithFlags(&b->side[s], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join[s], hipEventDisableTiming) != hipSuccess)
+      return fail(SCFT_E_HIP);
+  if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+    return fail(SCFT_E_HIP);
   if ((rc = dalloc(&b->d_desc, b->n_inst)) || (rc = dalloc(&b->d_x, b->total_x)) ||
       (rc = dalloc(&b->d_w, b->total_u)) || (rc = dalloc(&b->d_phit, b->total_u)) ||
       (rc = dalloc(&b->d_pw, b->total_pw)) || (rc = dalloc(&b->d_phi, b->total_u)) ||
@@ -330,6 +380,13 @@ extern "C" int scft_batch_destroy(scft_batch* b) {
   dfree(b->d_wcold);
   dfree(b->d_ms);
   dfree(b->d_iters);
+  if (b->h_active) (void)hipHostFree(b->h_active);
+  for (int s = 0; s < scft_batch::kSide; s++) {
+    if (b->side[s]) (void)hipStreamSynchronize(b->side[s]);
+    if (b->side[s]) (void)hipStreamDestroy(b->side[s]);
+    if (b->join[s]) (void)hipEventDestroy(b->join[s]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
   return SCFT_OK;
@@ -397,6 +454,44 @@ extern "C" int scft_batch_time_eval(scft_batch* b, const double* w, int reps,
   (void)hipEventDestroy(e1);
   if (ms_per_eval) *ms_per_eval = ms / reps;
   if (march_ms) *march_ms = ms / reps;
+  return SCFT_OK;
+}
+
+extern "C" int scft_batch_time_buckets(scft_batch* b, const double* w, int reps,
+                                       int max_buckets, int* n_buckets, double* ms,
+                                       double* bytes, int* n_items) {
+  if (!b || !w || reps < 1) return SCFT_E_ARG;
+  HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  const int nb = static_cast<int>(b->buckets.size());
+  int c = 0;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int i = nb - 1; i >= 0; i--) {
+    const Bucket& bk = b->buckets[i];
+    if (bk.items.empty()) continue;
+    const MarchArgs a = march_args(b, bk, i);
+    HIPCHK(launch_march(bk.K, bk.W, a, bk.grid, b->stream));  // warm-up
+    HIPCHK(hipEventRecord(e0, b->stream));
+    for (int r = 0; r < reps; r++) HIPCHK(launch_march(bk.K, bk.W, a, bk.grid, b->stream));
+    HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipEventSynchronize(e1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e0, e1));
+    if (c < max_buckets) {
+      if (ms) ms[c] = t / reps;
+      double by = 0.0;
+      for (int k : bk.items) by += eval_bytes(b->desc[k].n, b->desc[k].N);
+      if (bytes) bytes[c] = by;
+      if (n_items) n_items[c] = static_cast<int>(bk.items.size());
+    }
+    c++;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (n_buckets) *n_buckets = c;
   return SCFT_OK;
 }
 
@@ -478,7 +573,7 @@ extern "C" int scft_debug_slabs(scft_batch* b, int inst, double* out) {
     if (bk.items.empty()) continue;
     const int P = 64 * bk.W, K = bk.K, half = d.N / 2;
     std::vector<double> raw(static_cast<size_t>(half - 1) * P * K);
-    HIPCHK(hipMemcpy(raw.data(), b->d_slabs, raw.size() * sizeof(double),
+    HIPCHK(hipMemcpy(raw.data(), b->d_slabs + bk.slab_base, raw.size() * sizeof(double),
                      hipMemcpyDeviceToHost));
     for (int s = 1; s < half; s++)
       for (int u = 0; u < d.n; u++) {
@@ -504,6 +599,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->floor = 1e-8;
   o->fallback_factor = 0.0;
   o->check_every = 8;
+  o->progress_seconds = 0.0;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -528,33 +624,36 @@ static int ensure_mixer(scft_batch* b, int S) {
   return SCFT_OK;
 }
 
-extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, double* w,
-                                const double* w_cold, int* iters, unsigned* status,
-                                scft_solve_stats* stats) {
-  if (!b || !o || !w) return SCFT_E_ARG;
-  if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
-  const auto t0 = std::chrono::steady_clock::now();
-  SCFT_TRY(ensure_mixer(b, o->history + 1));
-  if (w_cold && !b->d_wcold) SCFT_TRY(dalloc(&b->d_wcold, b->total_u));
-  const int nb = static_cast<int>(b->buckets.size());
+extern "C" int scft_batch_upload(scft_batch* b, const double* w, const double* w_cold) {
+  if (!b || !w) return SCFT_E_ARG;
   HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
                         b->stream));
-  if (w_cold)
+  if (w_cold) {
+    if (!b->d_wcold) SCFT_TRY(dalloc(&b->d_wcold, b->total_u));
     HIPCHK(hipMemcpyAsync(b->d_wcold, w_cold, sizeof(double) * b->total_u,
                           hipMemcpyHostToDevice, b->stream));
-  std::vector<MixState> ms0(b->n_inst, MixState{0, 0, 0, 0, INFINITY});
-  HIPCHK(hipMemcpyAsync(b->d_ms, ms0.data(), sizeof(MixState) * b->n_inst,
-                        hipMemcpyHostToDevice, b->stream));
-  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
-  HIPCHK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * b->n_inst, b->stream));
+    b->has_wcold = true;
+  } else {
+    b->has_wcold = false;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCFT_OK;
+}
 
+extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts* o,
+                                         scft_solve_stats* stats) {
+  if (!b || !o) return SCFT_E_ARG;
+  if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
+  SCFT_TRY(ensure_mixer(b, o->history + 1));
+  const auto t0 = std::chrono::steady_clock::now();
+  const int nb = static_cast<int>(b->buckets.size());
   AndersonArgs a;
   a.desc = b->d_desc;
   a.w = b->d_w;
   a.r = b->d_r;
   a.rnorm = b->d_rnorm;
   a.phit = b->d_phit;
-  a.wcold = w_cold ? b->d_wcold : nullptr;
+  a.wcold = b->has_wcold ? b->d_wcold : nullptr;
   a.hw = b->d_hw;
   a.hr = b->d_hr;
   a.st = b->d_ms;
@@ -572,10 +671,18 @@ extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, doub
   a.reg = o->regularization;
   a.floor = o->floor;
   a.fallback_thresh = 0.0;
+  // fresh mixer state (a small kernel-free host copy of zeros is not needed:
+  // the per-instance state array is rewritten from a pinned staging copy)
+  if (static_cast<int>(b->ms0.size()) != b->n_inst)
+    b->ms0.assign(b->n_inst, MixState{0, 0, 0, 0, INFINITY});
+  HIPCHK(hipMemcpyAsync(b->d_ms, b->ms0.data(), sizeof(MixState) * b->n_inst,
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  HIPCHK(hipMemsetAsync(b->d_iters, 0, sizeof(int) * b->n_inst, b->stream));
 
   // first sweep eagerly: the warm-start fallback needs the median first residual
   SCFT_TRY(enqueue_sweep(b));
-  if (w_cold && o->fallback_factor > 0.0) {
+  if (b->has_wcold && o->fallback_factor > 0.0) {
     std::vector<double> rn(b->n_inst);
     HIPCHK(hipMemcpyAsync(rn.data(), b->d_rnorm, sizeof(double) * b->n_inst,
                           hipMemcpyDeviceToHost, b->stream));
@@ -601,32 +708,54 @@ extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, doub
   if (rc != SCFT_OK) return rc;
   HIPCHK(ce);
   HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  int* h_active = nullptr;
-  HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&h_active), sizeof(int)));
-  int sweeps = 1, check = std::max(1, o->check_every);
+  if (!b->h_active) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int)));
+  int sweeps = 1;
+  const int check = std::max(1, o->check_every);
+  double next_report = o->progress_seconds;
   for (;;) {
-    HIPCHK(hipMemcpyAsync(h_active, b->d_counters + nb, sizeof(int), hipMemcpyDeviceToHost,
+    HIPCHK(hipMemcpyAsync(b->h_active, b->d_counters + nb, sizeof(int), hipMemcpyDeviceToHost,
                           b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
-    if (*h_active == 0 || sweeps >= o->max_iter) break;
+    if (o->progress_seconds > 0.0) {
+      const double el =
+          std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      if (el >= next_report) {
+        fprintf(stderr, "scft: %.1f s, %d sweeps, %d of %d instances active\n", el, sweeps,
+                *b->h_active, b->n_inst);
+        next_report = el + o->progress_seconds;
+      }
+    }
+    if (*b->h_active == 0 || sweeps >= o->max_iter) break;
     for (int i = 0; i < check && sweeps < o->max_iter; i++, sweeps++)
       HIPCHK(hipGraphLaunch(exec, b->stream));
   }
   HIPCHK(hipStreamSynchronize(b->stream));
-  (void)hipHostFree(h_active);
   (void)hipGraphExecDestroy(exec);
   (void)hipGraphDestroy(graph);
+  const auto t1 = std::chrono::steady_clock::now();
+  b->last_sweeps = sweeps;
+  if (stats) {
+    stats->sweeps = sweeps;
+    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
+    stats->eval_seconds = 0.0;
+  }
+  return SCFT_OK;
+}
 
-  HIPCHK(hipMemcpyAsync(w, b->d_w, sizeof(double) * b->total_u, hipMemcpyDeviceToHost,
-                        b->stream));
-  std::vector<int> it(b->n_inst);
-  std::vector<unsigned> stv(b->n_inst);
-  HIPCHK(hipMemcpyAsync(it.data(), b->d_iters, sizeof(int) * b->n_inst, hipMemcpyDeviceToHost,
-                        b->stream));
+extern "C" int scft_batch_download(scft_batch* b, double* w, int* iters, unsigned* status,
+                                   scft_solve_stats* stats) {
+  if (!b) return SCFT_E_ARG;
+  if (w)
+    HIPCHK(hipMemcpyAsync(w, b->d_w, sizeof(double) * b->total_u, hipMemcpyDeviceToHost,
+                          b->stream));
+  std::vector<int> it(b->n_inst, 0);
+  std::vector<unsigned> stv(b->n_inst, 0);
+  if (b->d_iters)
+    HIPCHK(hipMemcpyAsync(it.data(), b->d_iters, sizeof(int) * b->n_inst,
+                          hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipMemcpyAsync(stv.data(), b->d_status, sizeof(unsigned) * b->n_inst,
                         hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
-  const auto t1 = std::chrono::steady_clock::now();
   int conv = 0;
   double ns = 0.0, by = 0.0;
   for (int k = 0; k < b->n_inst; k++) {
@@ -637,14 +766,27 @@ extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, doub
   if (iters) std::copy(it.begin(), it.end(), iters);
   if (status) std::copy(stv.begin(), stv.end(), status);
   if (stats) {
-    stats->sweeps = sweeps;
+    stats->sweeps = b->last_sweeps;
     stats->converged = conv;
-    stats->seconds = std::chrono::duration<double>(t1 - t0).count();
-    stats->eval_seconds = 0.0;
     stats->node_steps = ns;
     stats->bytes = by;
   }
   return conv == b->n_inst ? SCFT_OK : SCFT_E_NOCONV;
+}
+
+extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, double* w,
+                                const double* w_cold, int* iters, unsigned* status,
+                                scft_solve_stats* stats) {
+  if (!b || !o || !w) return SCFT_E_ARG;
+  SCFT_TRY(scft_batch_upload(b, w, w_cold));
+  scft_solve_stats st{};
+  SCFT_TRY(scft_batch_solve_resident(b, o, &st));
+  const double secs = st.seconds;
+  const int rc = scft_batch_download(b, w, iters, status, &st);
+  if (rc != SCFT_OK && rc != SCFT_E_NOCONV) return rc;
+  st.seconds = secs;
+  if (stats) *stats = st;
+  return rc;
 }
 
 // ------------------------------------------------------- drop-in evaluator

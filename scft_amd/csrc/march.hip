@@ -50,6 +50,23 @@ struct MarchShared {
 
 __device__ __forceinline__ void wg_barrier() { __syncthreads(); }
 
+// DPP lane moves on a double (two 32-bit halves).  CTRL is a gfx9 DPP
+// control: 0x110 + s = row_shr:s (lane p reads p - s of its 16-lane row),
+// 0x100 + s = row_shl:s (reads p + s).  Lanes whose source leaves the row
+// read 0 (bound_ctrl); every use multiplies them by a zero weight.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+template <int S>
+__device__ __forceinline__ double row_shr(double v) { return dpp_d<0x110 + S>(v); }
+template <int S>
+__device__ __forceinline__ double row_shl(double v) { return dpp_d<0x100 + S>(v); }
+
 // value of `v` in lane `l`, as a wave-uniform scalar
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);
@@ -288,30 +305,62 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   }
 
   // -------------------------------------------- scan constants of the march
+  // Each lane's chunk is an affine map: forward z_out = Af z_in + B, backward
+  // x_first = Gb x_next + C, with Af = prod(-alpha), Gb = prod(-gamma) fixed
+  // for the instance.  The per-step scans combine these maps over the wave:
+  // a Hillis-Steele scan inside each 16-lane row (DPP row_shr / row_shl by
+  // 1, 2, 4, 8), then the three row-to-row carries from readlane values and
+  // the fixed row products, then (several waves) a fold over wave totals in
+  // LDS.  All multipliers are precomputed here; a step costs one FMA per
+  // level plus the carries.
+  const int rl = lane & 15;   // lane within its row
+  const int row = lane >> 4;  // row within the wave
   double Af = 1.0, Gb = 1.0;
 #pragma unroll
   for (int t = 0; t < K; t++) {
     Af *= -al[t];
     Gb *= -ga[t];
   }
-  double mf[6], mb[6];
-#pragma unroll
-  for (int dd = 0; dd < 6; dd++) {
-    const int s = 1 << dd;
-    const double Ap = __shfl_up(Af, s);
-    const double Gp = __shfl_down(Gb, s);
-    mf[dd] = (lane >= s) ? Af : 0.0;
-    mb[dd] = (lane + s < 64) ? Gb : 0.0;
-    if (lane >= s) Af *= Ap;
-    if (lane + s < 64) Gb *= Gp;
+  double mf[4], mb[4];
+  {
+    double Ap, Gp;
+    Ap = row_shr<1>(Af); Gp = row_shl<1>(Gb);
+    mf[0] = (rl >= 1) ? Af : 0.0; mb[0] = (rl + 1 < 16) ? Gb : 0.0;
+    Af = (rl >= 1) ? Af * Ap : Af; Gb = (rl + 1 < 16) ? Gb * Gp : Gb;
+    Ap = row_shr<2>(Af); Gp = row_shl<2>(Gb);
+    mf[1] = (rl >= 2) ? Af : 0.0; mb[1] = (rl + 2 < 16) ? Gb : 0.0;
+    Af = (rl >= 2) ? Af * Ap : Af; Gb = (rl + 2 < 16) ? Gb * Gp : Gb;
+    Ap = row_shr<4>(Af); Gp = row_shl<4>(Gb);
+    mf[2] = (rl >= 4) ? Af : 0.0; mb[2] = (rl + 4 < 16) ? Gb : 0.0;
+    Af = (rl >= 4) ? Af * Ap : Af; Gb = (rl + 4 < 16) ? Gb * Gp : Gb;
+    Ap = row_shr<8>(Af); Gp = row_shl<8>(Gb);
+    mf[3] = (rl >= 8) ? Af : 0.0; mb[3] = (rl + 8 < 16) ? Gb : 0.0;
+    Af = (rl >= 8) ? Af * Ap : Af; Gb = (rl + 8 < 16) ? Gb * Gp : Gb;
   }
-  const double Apre_m1 = __shfl_up(Af, 1);   // prod over wave start .. lane-1
-  const double Gsuf_p1 = __shfl_down(Gb, 1);  // prod over lane+1 .. wave end
-  double gl = __shfl_up(ga[K - 1], 1);         // gamma of the row left of me
+  // Af = product over the row from its first lane to this lane, Gb = product
+  // from this lane to the row's last lane; row totals as uniform scalars
+  const double Arow0 = readlane_d(Af, 15), Arow1 = readlane_d(Af, 31);
+  const double Arow2 = readlane_d(Af, 47), Arow3 = readlane_d(Af, 63);
+  const double Grow0 = readlane_d(Gb, 0), Grow1 = readlane_d(Gb, 16);
+  const double Grow2 = readlane_d(Gb, 32), Grow3 = readlane_d(Gb, 48);
+  // DPP reads are made with every lane active: a DPP move in a conditional
+  // arm runs under a partial EXEC mask and reads 0 from disabled lanes.
+  const double Af_sh = row_shr<1>(Af), Gb_sh = row_shl<1>(Gb);
+  const double Ashift = (rl > 0) ? Af_sh : 0.0;   // row product to lane-1
+  const double Gshift = (rl < 15) ? Gb_sh : 0.0;  // row product from lane+1
+  // products over whole rows before / after this lane's row
+  const double Abefore = (row == 0) ? 1.0 : (row == 1) ? Arow0
+                         : (row == 2) ? Arow0 * Arow1 : Arow0 * Arow1 * Arow2;
+  const double Gafter = (row == 3) ? 1.0 : (row == 2) ? Grow3
+                        : (row == 1) ? Grow2 * Grow3 : Grow1 * Grow2 * Grow3;
+  // wave prefix over lanes 0..lane-1 and suffix over lane+1..63 (wave folds)
+  const double Apre_m1 = ((rl > 0) ? Ashift : 1.0) * Abefore;
+  const double Gsuf_p1 = ((rl < 15) ? Gshift : 1.0) * Gafter;
+  double gl = __shfl_up(ga[K - 1], 1);  // gamma of the row left of me
   if constexpr (W > 1) {
-    sh.Aw[wave] = readlane_d(Af, 63);
+    sh.Aw[wave] = Arow0 * Arow1 * Arow2 * Arow3;
     sh.gx[wave] = readlane_d(ga[K - 1], 63);
-    sh.Gw[wave] = readlane_d(Gb, 0);
+    sh.Gw[wave] = Grow0 * Grow1 * Grow2 * Grow3;
     wg_barrier();
     const double gx_prev = (wave > 0) ? sh.gx[wave - 1] : 0.0;
     gl = (lane == 0) ? gx_prev : gl;
@@ -343,20 +392,23 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     double B = 0.0;
 #pragma unroll
     for (int t = 0; t < K; t++) B = rp[t] - al[t] * B;
-#pragma unroll
-    for (int dd = 0; dd < 6; dd++) {
-      const double Bp = __shfl_up(B, 1 << dd);
-      B = fma(mf[dd], Bp, B);
-    }
-    double zin = __shfl_up(B, 1);
+    B = fma(mf[0], row_shr<1>(B), B);
+    B = fma(mf[1], row_shr<2>(B), B);
+    B = fma(mf[2], row_shr<4>(B), B);
+    B = fma(mf[3], row_shr<8>(B), B);
+    // carries into rows 1..3: the prefix up to the last lane of the row before
+    const double c1 = readlane_d(B, 15);
+    const double c2 = fma(Arow1, c1, readlane_d(B, 31));
+    const double c3 = fma(Arow2, c2, readlane_d(B, 47));
+    const double cm = (row == 0) ? 0.0 : (row == 1) ? c1 : (row == 2) ? c2 : c3;
+    const double B_sh = row_shr<1>(B);
+    double zin = (rl == 0) ? cm : fma(Ashift, cm, B_sh);
     if constexpr (W > 1) {
-      sh.Bw[wave] = readlane_d(B, 63);
+      sh.Bw[wave] = fma(Arow3, c3, readlane_d(B, 63));
       wg_barrier();
       double Z = 0.0;
       for (int v = 0; v < wave; v++) Z = fma(sh.Aw[v], Z, sh.Bw[v]);
-      zin = (lane > 0) ? fma(Apre_m1, Z, zin) : Z;
-    } else {
-      zin = (lane > 0) ? zin : 0.0;
+      zin = fma(Apre_m1, Z, zin);
     }
     {
       double z = zin;
@@ -370,20 +422,23 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     double Cc = 0.0;
 #pragma unroll
     for (int t = K - 1; t >= 0; t--) Cc = rp[t] - ga[t] * Cc;
-#pragma unroll
-    for (int dd = 0; dd < 6; dd++) {
-      const double Cp = __shfl_down(Cc, 1 << dd);
-      Cc = fma(mb[dd], Cp, Cc);
-    }
-    double xin = __shfl_down(Cc, 1);
+    Cc = fma(mb[0], row_shl<1>(Cc), Cc);
+    Cc = fma(mb[1], row_shl<2>(Cc), Cc);
+    Cc = fma(mb[2], row_shl<4>(Cc), Cc);
+    Cc = fma(mb[3], row_shl<8>(Cc), Cc);
+    // carries into rows 0..2: the suffix from the first lane of the row after
+    const double e2 = readlane_d(Cc, 48);
+    const double e1 = fma(Grow2, e2, readlane_d(Cc, 32));
+    const double e0 = fma(Grow1, e1, readlane_d(Cc, 16));
+    const double em = (row == 3) ? 0.0 : (row == 2) ? e2 : (row == 1) ? e1 : e0;
+    const double C_sh = row_shl<1>(Cc);
+    double xin = (rl == 15) ? em : fma(Gshift, em, C_sh);
     if constexpr (W > 1) {
-      sh.Cw[wave] = readlane_d(Cc, 0);
+      sh.Cw[wave] = fma(Grow0, e0, readlane_d(Cc, 0));
       wg_barrier();
       double X = 0.0;
       for (int v = W - 1; v > wave; v--) X = fma(sh.Gw[v], X, sh.Cw[v]);
-      xin = (lane < 63) ? fma(Gsuf_p1, X, xin) : X;
-    } else {
-      xin = (lane < 63) ? xin : 0.0;
+      xin = fma(Gsuf_p1, X, xin);
     }
     {
       double xx = xin;
