@@ -40,11 +40,18 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# The mixer of both sides: history 10, log(phi/phi_t) mixing residual
+# preconditioned by I + 0.5 M_L^{-1} K, beta 1, no step cap, a NaN evaluation
+# halves the step (backtrack factor 1e30: NaN only).
+SOLVER = dict(history=10, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8, precond=0.5,
+              backtrack=1e30, max_halvings=30)
+SOLVER_TEXT = ("device Anderson, history 10, log mixing preconditioned by I + 0.5 M^-1 K, "
+               "beta 1.0, no step cap, NaN backtracking")
+
+
 def solver_opts(S, max_iter):
-    # history 10, log(phi/phi_t) mixing residual, beta 1, step cap 2 (inf-norm)
-    return S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=max_iter, step_cap=2.0,
-                           transform=S.MIX_LOG, floor=1e-8, check_every=16,
-                           progress_seconds=20.0)
+    return S.anderson_opts(max_iter=max_iter, transform=S.MIX_LOG, check_every=4,
+                           progress_seconds=20.0, **SOLVER)
 
 
 def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
@@ -69,8 +76,7 @@ def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
             break
         idx.append(int(k))
         est += work[k] * c
-    opts = O.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=max_iter, step_cap=2.0,
-                           transform=O.MIX_LOG, floor=1e-8)
+    opts = O.anderson_opts(max_iter=max_iter, transform=O.MIX_LOG, **SOLVER)
     t0 = time.perf_counter()
     out = O.batch_solve([len(cfg["xs"][k]) for k in idx], [cfg["Ns"][k] for k in idx],
                         [cfg["xs"][k] for k in idx], [cfg["pts"][k] for k in idx],
@@ -87,13 +93,14 @@ def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n-inst", type=int, default=4096)
-    ap.add_argument("--max-iter", type=int, default=20000)
+    ap.add_argument("--max-iter", type=int, default=500)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (per thread) to time")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-file", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -109,6 +116,10 @@ def main():
     w0 = [w[1:] for w in cfg["w0"]]
     b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN,
                 tune_file=args.tune_file)
+    if not args.no_autotune:  # variant choice per bucket: setup, outside the timed region
+        t0 = time.perf_counter()
+        b.autotune(reps=2)
+        log(f"bench: autotune {time.perf_counter() - t0:.1f} s")
     log(f"bench: buckets (n_max, K, W) = {b.tuning()}")
 
     # roofline of the dominant march launch (HIP events on the launch stream)
@@ -156,7 +167,7 @@ def main():
                         "N in {512,1024,2048,4096}, L in [2,8], cold start, tol 1e-6",
             "n_inst": args.n_inst,
             "start": "cold",
-            "solver": "device Anderson, history 10, log mixing, beta 1.0, step cap 2.0",
+            "solver": SOLVER_TEXT,
             "converged": int(res["converged"]),
             "mean_iters": round(float(np.mean(res["iters"])), 1),
             "max_iters": int(np.max(res["iters"])),
@@ -173,7 +184,7 @@ def main():
         },
     }
     if not args.no_cpu:
-        threads = len(os.sched_getaffinity(0))
+        threads = min(16, len(os.sched_getaffinity(0)))  # one GPU's share of the host
         cb = cpu_baseline(cfg, res["iters"], args.max_iter, threads, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": round(cb["value"], 4), "unit": "solves/s", "cores": threads,

@@ -44,6 +44,7 @@ extern "C" {
 #define SCFT_ST_PIVOT 4u
 #define SCFT_ST_MAXITER 8u
 #define SCFT_ST_FALLBACK 16u /* warm start rejected, restarted cold */
+#define SCFT_ST_FAILED 32u   /* evaluation failed (NaN / pivot) beyond recovery */
 
 /* ---- library lifetime ---------------------------------------------------
  * Replaces PetscInitialize/PetscFinalize around the evaluator
@@ -150,6 +151,13 @@ typedef struct scft_anderson_opts {
                              fallback_factor * median restart cold (<=0: off) */
   int check_every;        /* host polls the active count every k sweeps */
   double progress_seconds;/* > 0: print a progress line to stderr this often */
+  double precond;         /* a >= 0: mix g = (I + a M_L^{-1} K) f, the lumped-
+                             mass Laplacian of the unknowns' grid with natural
+                             ends (0: off, the reference's plain mixing) */
+  double backtrack;       /* > 0: a NaN / failed evaluation, or one whose
+                             ||r|| exceeds backtrack * the last accepted one,
+                             halves the last step and clears the history */
+  int max_halvings;       /* consecutive halvings before the instance fails */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

@@ -55,7 +55,8 @@ class AndersonOpts(C.Structure):
     _fields_ = [("history", C.c_int), ("beta", C.c_double), ("tol", C.c_double),
                 ("max_iter", C.c_int), ("step_cap", C.c_double),
                 ("restart_factor", C.c_double), ("regularization", C.c_double),
-                ("transform", C.c_int), ("floor", C.c_double)]
+                ("transform", C.c_int), ("floor", C.c_double), ("precond", C.c_double),
+                ("pstencil", C.c_void_p), ("backtrack", C.c_double), ("max_halvings", C.c_int)]
 
 
 class AndersonStats(C.Structure):
@@ -112,10 +113,10 @@ def lib():
         L.orc_broydn.argtypes = [_dp, C.c_int, C.c_void_p, C.c_void_p, C.c_double,
                                  C.POINTER(C.c_int), _dp, _dp, _dp, C.POINTER(BroydnStats)]
         L.orc_solve_anderson.argtypes = [C.POINTER(OrcProblem), _dp, C.POINTER(AndersonOpts),
-                                         C.c_int, C.c_double, C.POINTER(AndersonStats)]
+                                         C.POINTER(AndersonStats)]
         L.orc_batch_solve.argtypes = [C.c_int, _ip, _ip, _lp, _dp, _dp, _dp, C.c_int, C.c_int,
-                                      C.POINTER(AndersonOpts), C.c_int, C.c_double, C.c_int,
-                                      _ip, _ip, _dp]
+                                      C.POINTER(AndersonOpts), C.c_int, _ip, _ip, _dp]
+        L.orc_precond_stencil.argtypes = [_dp, C.c_int, C.c_int, _dp]
         L.orc_batch_eval.argtypes = [C.c_int, _ip, _ip, _lp, _dp, _dp, _dp, C.c_int, C.c_int,
                                      C.c_int, _dp, _dp, _dp]
         L.orc_global_setup.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_long]
@@ -292,26 +293,24 @@ MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 
 
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=2000, step_cap=0.5,
-                  restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8):
+                  restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
+                  precond=0.0, backtrack=0.0, max_halvings=30):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
-                        transform, floor)
+                        transform, floor, precond, None, backtrack, max_halvings)
 
 
-def solve_anderson(x, phi_t, N, w_full, opts=None, bc=BC_DN, cmat=C_ROWSCALED,
-                   precond=False, pfloor=1e-3, trace_len=0):
+def solve_anderson(x, phi_t, N, w_full, opts=None, bc=BC_DN, cmat=C_ROWSCALED, trace_len=0):
     p = make_problem(x, phi_t, N, bc, cmat)
     w = _f64(w_full).copy()
     opts = opts or anderson_opts()
     tr = np.zeros(max(trace_len, 1))
     st = AndersonStats(0, 0, 0.0, tr.ctypes.data_as(C.POINTER(C.c_double)), trace_len)
-    rc = lib().orc_solve_anderson(C.byref(p), w, C.byref(opts), 1 if precond else 0,
-                                  pfloor, C.byref(st))
+    rc = lib().orc_solve_anderson(C.byref(p), w, C.byref(opts), C.byref(st))
     return dict(rc=rc, w=w, iterations=st.iterations, restarts=st.restarts,
                 final_err=st.final_err, trace=tr[:min(trace_len, st.iterations)])
 
 
-def batch_solve(Ms, Ns, xs, phits, ws, opts, bc=BC_DN, cmat=C_ROWSCALED, precond=False,
-                pfloor=1e-3, threads=0):
+def batch_solve(Ms, Ns, xs, phits, ws, opts, bc=BC_DN, cmat=C_ROWSCALED, threads=0):
     """Solve independent instances over OpenMP threads (the CPU baseline)."""
     Ms = np.ascontiguousarray(Ms, dtype=np.int32)
     Ns = np.ascontiguousarray(Ns, dtype=np.int32)
@@ -323,8 +322,8 @@ def batch_solve(Ms, Ns, xs, phits, ws, opts, bc=BC_DN, cmat=C_ROWSCALED, precond
     iters = np.zeros(len(Ms), dtype=np.int32)
     status = np.zeros(len(Ms), dtype=np.int32)
     ferr = np.zeros(len(Ms))
-    lib().orc_batch_solve(len(Ms), Ms, Ns, off, x, pt, w, bc, cmat, C.byref(opts),
-                          1 if precond else 0, pfloor, threads, iters, status, ferr)
+    lib().orc_batch_solve(len(Ms), Ms, Ns, off, x, pt, w, bc, cmat, C.byref(opts), threads,
+                          iters, status, ferr)
     ws_out = [w[o:o + m] for o, m in zip(off, Ms)]
     return dict(w=ws_out, iters=iters, status=status, final_err=ferr)
 

@@ -61,15 +61,21 @@ int orc_residual_dealii_cb(int n, const double *w, double *r, void *ctx) {
 }
 
 int orc_solve_anderson(const orc_problem *p, double *w_full,
-                       const orc_anderson_opts *o, int precond,
-                       double pfloor, orc_anderson_stats *st) {
+                       const orc_anderson_opts *o, orc_anderson_stats *st) {
   orc_eval_ctx c;
   if (orc_ctx_init(&c, p, o->beta) != ORC_OK) return ORC_E_ALLOC;
   int n = orc_n_unknowns(p->M, p->bc);
   memcpy(c.wfull, w_full, sizeof(double) * p->M);
-  (void)precond;
-  (void)pfloor;
-  int rc = orc_anderson(w_full + 1, n, orc_residual_cb, &c, o, p->phi_t + 1, st);
+  orc_anderson_opts oo = *o;
+  double *ps = NULL;
+  if (oo.precond > 0.0 && !oo.pstencil) { /* stencil of this instance's mesh */
+    ps = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    if (!ps) { orc_ctx_free(&c); return ORC_E_ALLOC; }
+    orc_precond_stencil(p->x, p->M, p->bc, ps);
+    oo.pstencil = ps;
+  }
+  int rc = orc_anderson(w_full + 1, n, orc_residual_cb, &c, &oo, p->phi_t + 1, st);
+  free(ps);
   orc_ctx_free(&c);
   return rc;
 }
@@ -80,9 +86,8 @@ int orc_solve_anderson(const orc_problem *p, double *w_full,
  * evaluation count and the solver's return code. */
 int orc_batch_solve(int n_inst, const int *M, const int *N, const long *xoff,
                     const double *x, const double *phi_t, double *w, int bc,
-                    int cmat, const orc_anderson_opts *o, int precond,
-                    double pfloor, int threads, int *iters, int *status,
-                    double *final_err) {
+                    int cmat, const orc_anderson_opts *o, int threads,
+                    int *iters, int *status, double *final_err) {
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #else
@@ -92,7 +97,7 @@ int orc_batch_solve(int n_inst, const int *M, const int *N, const long *xoff,
   for (int k = 0; k < n_inst; k++) {
     orc_problem p = {M[k], N[k], bc, cmat, x + xoff[k], phi_t + xoff[k]};
     orc_anderson_stats st = {0, 0, 0.0, NULL, 0};
-    status[k] = orc_solve_anderson(&p, w + xoff[k], o, precond, pfloor, &st);
+    status[k] = orc_solve_anderson(&p, w + xoff[k], o, &st);
     iters[k] = st.iterations;
     if (final_err) final_err[k] = st.final_err;
   }

@@ -26,14 +26,12 @@ void orc_fixed_point_cb(int n, const double *x, double *xnew, void *ctx);
 int orc_residual_dealii_cb(int n, const double *w, double *r, void *ctx);
 
 int orc_solve_anderson(const orc_problem *p, double *w_full,
-                       const orc_anderson_opts *o, int precond,
-                       double pfloor, orc_anderson_stats *st);
+                       const orc_anderson_opts *o, orc_anderson_stats *st);
 
 int orc_batch_solve(int n_inst, const int *M, const int *N, const long *xoff,
                     const double *x, const double *phi_t, double *w, int bc,
-                    int cmat, const orc_anderson_opts *o, int precond,
-                    double pfloor, int threads, int *iters, int *status,
-                    double *final_err);
+                    int cmat, const orc_anderson_opts *o, int threads,
+                    int *iters, int *status, double *final_err);
 
 int orc_batch_eval(int n_inst, const int *M, const int *N, const long *xoff,
                    const double *x, const double *phi_t, const double *w,

@@ -9,6 +9,7 @@
  * reference's are 1-based Numerical-Recipes offsets.
  */
 #include "oracle_solvers.h"
+#include "scft_oracle.h"
 
 #include <math.h>
 #include <stdio.h>
@@ -160,27 +161,58 @@ out:
 /*   the step w_{k+1} - w_k is scaled down to ||.||_inf <= cap                 */
 /*   history is cleared when ||r_k|| > restart * best so far, or U singular    */
 /* ------------------------------------------------------------------------ */
+int orc_precond_stencil(const double *x, int M, int bc, double *st) {
+  int n = orc_n_unknowns(M, bc);
+  if (n < 1) return ORC_E_ARG;
+  if (n == 1) { st[0] = st[1] = st[2] = 0.0; return ORC_OK; }
+  for (int u = 0; u < n; u++) {
+    int i = u + 1; /* node of unknown u */
+    double hl = (u > 0) ? x[i] - x[i - 1] : 0.0;
+    double hr = (u + 1 < n) ? x[i + 1] - x[i] : 0.0;
+    double m = 0.5 * (hl + hr);
+    st[3 * u + 0] = (u > 0) ? (1.0 / hl) / m : 0.0;
+    st[3 * u + 2] = (u + 1 < n) ? (1.0 / hr) / m : 0.0;
+    st[3 * u + 1] = st[3 * u + 0] + st[3 * u + 2];
+  }
+  return ORC_OK;
+}
+
 int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
                  const orc_anderson_opts *o, const double *phi_t_unk,
                  orc_anderson_stats *st) {
+  if (o->precond > 0.0 && !o->pstencil) return 3;
   int mh = o->history;
   int S = mh + 1;
   double *wh = (double *)calloc((size_t)n * S, sizeof(double));
   double *rh = (double *)calloc((size_t)n * S, sizeof(double));
   double *r = (double *)malloc(sizeof(double) * n);
   double *wn = (double *)malloc(sizeof(double) * n);
+  double *wa = (double *)malloc(sizeof(double) * n); /* last accepted iterate */
   double U[64 * 64], bb[64];
-  int stored = 0, head = 0, k, rc = 1, restarts = 0;
-  double best = INFINITY, err = INFINITY;
+  int stored = 0, head = 0, k, rc = 1, restarts = 0, halvings = 0;
+  double best = INFINITY, err = INFINITY, eacc = INFINITY;
   for (k = 0; k < o->max_iter; k++) {
-    if (F(n, w, r, ctx) != 0) { rc = 2; break; }
+    int ferr = F(n, w, r, ctx);
     err = 0.0;
     for (int i = 0; i < n; i++) {
       if (isnan(r[i])) { err = NAN; break; }
       err = fmax(err, fabs(r[i]));
     }
     if (st && st->err_trace && k < st->trace_len) st->err_trace[k] = err;
-    if (isnan(err)) { rc = 2; break; }
+    /* backtracking: halve the step that led here, drop the history */
+    if (o->backtrack > 0.0 && k > 0 &&
+        (ferr != 0 || isnan(err) || err > o->backtrack * eacc)) {
+      if (halvings >= o->max_halvings) { rc = 2; break; }
+      for (int i = 0; i < n; i++) w[i] = wa[i] + 0.5 * (w[i] - wa[i]);
+      halvings++;
+      restarts++;
+      stored = 0;
+      continue;
+    }
+    if (ferr != 0 || isnan(err)) { rc = 2; break; }
+    halvings = 0;
+    eacc = err;
+    memcpy(wa, w, sizeof(double) * n);
     if (err <= o->tol) { rc = 0; break; }
     if (err > o->restart_factor * best && stored > 0) {
       stored = 0;
@@ -194,6 +226,18 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
         if (o->transform == ORC_MIX_RELATIVE) r[i] = r[i] / pt;
         else r[i] = log(fmax(pt + r[i], o->floor * 1e-3) / pt);
       }
+    }
+    /* preconditioned mixing residual g = f + a M_L^{-1} K f */
+    if (o->precond > 0.0) {
+      const double *ps = o->pstencil;
+      double fl = 0.0, fc = r[0];
+      for (int i = 0; i < n; i++) {
+        double fr = (i + 1 < n) ? r[i + 1] : 0.0;
+        wn[i] = fc + o->precond * (ps[3 * i + 1] * fc - ps[3 * i] * fl - ps[3 * i + 2] * fr);
+        fl = fc;
+        fc = fr;
+      }
+      memcpy(r, wn, sizeof(double) * n);
     }
     /* push (w_k, r_k) */
     memcpy(wh + (size_t)n * head, w, sizeof(double) * n);
@@ -252,6 +296,7 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
   free(rh);
   free(r);
   free(wn);
+  free(wa);
   return rc;
 }
 

@@ -33,7 +33,20 @@ typedef struct orc_anderson_opts {
   double regularization;/* U += reg * trace(U) * I before the solve */
   int transform;        /* ORC_MIX_PLAIN / _RELATIVE / _LOG */
   double floor;         /* phi_t floor used by the transforms */
+  double precond;       /* a >= 0: mix g = (I + a M_L^{-1} K) f instead of f */
+  const double *pstencil; /* 3 n doubles (l, d, r per unknown) when a > 0 */
+  double backtrack;     /* > 0: an evaluation that is NaN or whose error
+                           exceeds backtrack * (last accepted error) halves
+                           the last step and clears the history */
+  int max_halvings;     /* consecutive halvings allowed before giving up */
 } orc_anderson_opts;
+
+/* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements
+ * between consecutive unknowns, lumped mass, natural rows at both ends: the
+ * mixing residual is not zero at the walls):
+ * (M_L^{-1} K f)_u = d_u f_u - l_u f_{u-1} - r_u f_{u+1}, d_u = l_u + r_u.
+ * st receives 3 n doubles. */
+int orc_precond_stencil(const double *x, int M, int bc, double *st);
 
 enum { ORC_MIX_PLAIN = 0, ORC_MIX_RELATIVE = 1, ORC_MIX_LOG = 2 };
 

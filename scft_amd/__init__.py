@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(HERE, "libscft_hip.so")
 SCFT_OK = 0
 BC_DD, BC_DN = 0, 1
 C_ROWSCALED, C_CONSISTENT = 0, 1
-ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK = 1, 2, 4, 8, 16
+ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK, ST_FAILED = 1, 2, 4, 8, 16, 32
 MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 L_REF = 3.72374357332160
 TAU = 0.530252230020752
@@ -60,7 +60,8 @@ class AndersonOpts(C.Structure):
                 ("restart_factor", C.c_double), ("regularization", C.c_double),
                 ("transform", C.c_int), ("floor", C.c_double),
                 ("fallback_factor", C.c_double), ("check_every", C.c_int),
-                ("progress_seconds", C.c_double)]
+                ("progress_seconds", C.c_double), ("precond", C.c_double),
+                ("backtrack", C.c_double), ("max_halvings", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -161,9 +162,11 @@ def local_iteration() -> int:
 # ------------------------------------------------------------------ batches
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
-                  fallback_factor=0.0, check_every=8, progress_seconds=0.0):
+                  fallback_factor=0.0, check_every=8, progress_seconds=0.0, precond=0.0,
+                  backtrack=0.0, max_halvings=30):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
-                        transform, floor, fallback_factor, check_every, progress_seconds)
+                        transform, floor, fallback_factor, check_every, progress_seconds,
+                        precond, backtrack, max_halvings)
 
 
 class Batch:

@@ -13,6 +13,16 @@
 // exceeds restart_factor times the best so far, and a warm start whose first
 // residual is fallback_factor times the batch median is replaced by the cold
 // start.  Convergence is judged on the plain residual phi - phi_t.
+//
+// Extensions (off by default, not in the reference):
+//  * preconditioned mixing residual g = f + a M_L^{-1} K f, with M_L^{-1} K
+//    the lumped-mass linear-element Laplacian of the unknowns' grid with
+//    natural ends.  phi responds to a field perturbation of wavenumber k like
+//    the Debye function, ~2/k^2 for k >> 1 (contour length 1, q_s = q_xx),
+//    so I + K/2 undoes the response at short wavelengths;
+//  * backtracking: a failed evaluation (NaN / zero pivot) or one whose ||r||
+//    exceeds backtrack times the last accepted one halves the step toward the
+//    last accepted iterate (ring slot head - 1) and clears the history.
 #include "scft_internal.h"
 
 namespace scft {
@@ -38,7 +48,8 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   __shared__ int s_m, s_mode;
   __shared__ double s_scale;
 
-  if (__builtin_amdgcn_readfirstlane(a.status[inst]) & ST_DONE_MASK) return;
+  const unsigned st0 = __builtin_amdgcn_readfirstlane(a.status[inst]);
+  if (st0 & (ST_CONVERGED | ST_MAXITER | ST_FAILED)) return;
   const InstDesc d = a.desc[inst];
   const int n = __builtin_amdgcn_readfirstlane(d.n);
   const int S = a.history + 1;
@@ -54,8 +65,13 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // ------------------------------------------------------------ stop tests
   if (tid == 0) {
     s_mode = 0;
-    if (!(err == err)) {
-      s_mode = 1;  // NaN already flagged by the march
+    const bool bad_eval = !(err == err) || (st0 & (ST_NAN | ST_PIVOT));
+    const bool have_acc = a.backtrack > 0.0 && ms.eacc < INFINITY;
+    const bool can_halve = have_acc && ms.halvings < a.max_halvings;
+    if (bad_eval) {
+      s_mode = can_halve ? 5 : 1;  // NaN / pivot flagged by the march
+    } else if (have_acc && err > a.backtrack * ms.eacc) {
+      s_mode = can_halve ? 5 : 1;
     } else if (err <= a.tol) {
       s_mode = 2;
     } else if (ms.k == 0 && a.fallback_thresh > 0.0 && err > a.fallback_thresh &&
@@ -69,6 +85,7 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   const int mode = __builtin_amdgcn_readfirstlane(s_mode);
   if (mode == 1 || mode == 2 || mode == 4) {
     if (tid == 0) {
+      if (mode == 1) a.status[inst] |= ST_FAILED;
       if (mode == 2) a.status[inst] |= ST_CONVERGED;
       if (mode == 4) a.status[inst] |= ST_MAXITER;
       a.iters[inst] = k;
@@ -86,11 +103,32 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
       ms.stored = 0;
       ms.head = 0;
       ms.best = INFINITY;
+      ms.eacc = INFINITY;
+      ms.halvings = 0;
       a.st[inst] = ms;
       atomicAdd(a.active, 1);
     }
     return;
   }
+  if (mode == 5) {  // halve the last step toward the last accepted iterate
+    const int sa = __builtin_amdgcn_readfirstlane((ms.head - 1 + S) % S);
+    for (int u = tid; u < n; u += AM_THREADS) {
+      const double wa = hw[(long long)sa * n + u];
+      w[u] = wa + 0.5 * (w[u] - wa);
+    }
+    if (tid == 0) {
+      a.status[inst] = st0 & ~(ST_NAN | ST_PIVOT);
+      ms.k = k;
+      ms.halvings++;
+      ms.restarts++;
+      ms.stored = 0;
+      a.st[inst] = ms;
+      atomicAdd(a.active, 1);
+    }
+    return;
+  }
+  ms.halvings = 0;
+  ms.eacc = err;
 
   // ----------------------------------------------- history bookkeeping
   int stored = ms.stored;
@@ -102,10 +140,25 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   const int cur = __builtin_amdgcn_readfirstlane(ms.head);
   int m = __builtin_amdgcn_readfirstlane(stored < a.history ? stored : a.history);
 
-  // push (w_k, rt_k) into slot `cur`
+  // push (w_k, g_k) into slot `cur`
+  const double* __restrict__ xn = a.x + d.xo;
+  const bool pc = a.precond > 0.0 && n > 1;
   for (int u = tid; u < n; u += AM_THREADS) {
     hw[(long long)cur * n + u] = w[u];
-    hr[(long long)cur * n + u] = mix_residual(r[u], pt[u], a.transform, a.floor);
+    double f = mix_residual(r[u], pt[u], a.transform, a.floor);
+    if (pc) {
+      const int i = u + 1;  // node of unknown u
+      const bool has_l = u > 0, has_r = u + 1 < n;
+      const double fl = has_l ? mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor) : 0.0;
+      const double fr = has_r ? mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor) : 0.0;
+      const double hl = has_l ? xn[i] - xn[i - 1] : 0.0;
+      const double hrr = has_r ? xn[i + 1] - xn[i] : 0.0;
+      const double mm = 0.5 * (hl + hrr);
+      const double cl = has_l ? (1.0 / hl) / mm : 0.0;
+      const double cr = has_r ? (1.0 / hrr) / mm : 0.0;
+      f = f + a.precond * ((cl + cr) * f - cl * fl - cr * fr);
+    }
+    hr[(long long)cur * n + u] = f;
   }
   __syncthreads();
 

@@ -119,7 +119,7 @@ static double eval_bytes(int n, int N) {
 }
 
 // Every non-empty bucket owns a scratch region (the buckets of one sweep run
-// concurrently): grid_b workgroups x (N_max/2 - 1) slabs x 64 W K doubles.
+// concurrently): grid_b workgroups x N_max/2 slabs x 64 W K doubles.
 // Buckets are spread over the main and side streams, largest work first.
 static int size_scratch(scft_batch* b) {
   long long need = 0;
@@ -132,9 +132,9 @@ static int size_scratch(scft_batch* b) {
       return SCFT_E_ARG;
     }
     const int P = 64 * bk.W;
-    const int mg = march_max_grid(bk.K, bk.W);
+    const int mg = march_max_grid(bk.K, bk.W, bk.Nmax / 2);
     bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), mg));
-    bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2 - 1)) * P * bk.K;
+    bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2)) * P * bk.K;
     bk.slab_base = need;
     need += bk.slab_stride * bk.grid;
     double work = 0.0;
@@ -192,6 +192,7 @@ static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_id
   a.status = b->d_status;
   a.slabs = b->d_slabs + bk.slab_base;
   a.slab_stride = bk.slab_stride;
+  a.half_max = bk.Nmax / 2;
   a.items = bk.d_items;
   a.n_items = static_cast<int>(bk.items.size());
   (void)bucket_idx;
@@ -252,7 +253,10 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
   b->uoff.resize(b->n_inst);
   for (int k = 0; k < b->n_inst; k++) {
     const int M = dsc->M[k], N = dsc->N[k];
-    if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > 4096 || N < 16 || (N & (N - 1)) != 0) {
+    // N: a power of two (Romberg) in [16, 16384] (the N/2 + 1 pairing weights
+    // of a bucket are held in LDS)
+    if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > 4096 || N < 16 || N > 16384 ||
+        (N & (N - 1)) != 0) {
       snprintf(g_errmsg, sizeof g_errmsg, "instance %d: M=%d N=%d unsupported", k, M, N);
       delete b;
       return SCFT_E_ARG;
@@ -572,14 +576,14 @@ extern "C" int scft_debug_slabs(scft_batch* b, int inst, double* out) {
   for (auto& bk : b->buckets) {
     if (bk.items.empty()) continue;
     const int P = 64 * bk.W, K = bk.K, half = d.N / 2;
-    std::vector<double> raw(static_cast<size_t>(half - 1) * P * K);
+    std::vector<double> raw(static_cast<size_t>(half) * P * K);
     HIPCHK(hipMemcpy(raw.data(), b->d_slabs + bk.slab_base, raw.size() * sizeof(double),
                      hipMemcpyDeviceToHost));
     for (int s = 1; s < half; s++)
       for (int u = 0; u < d.n; u++) {
         const int p = u / K, t = u % K;
         out[static_cast<size_t>(s - 1) * d.n + u] =
-            raw[static_cast<size_t>(s - 1) * P * K + static_cast<size_t>(t) * P + p];
+            raw[static_cast<size_t>(s) * P * K + static_cast<size_t>(t) * P + p];
       }
     return SCFT_OK;
   }
@@ -600,6 +604,9 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->fallback_factor = 0.0;
   o->check_every = 8;
   o->progress_seconds = 0.0;
+  o->precond = 0.0;
+  o->backtrack = 0.0;
+  o->max_halvings = 30;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -644,6 +651,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
                                          scft_solve_stats* stats) {
   if (!b || !o) return SCFT_E_ARG;
   if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
+  if (!(o->precond >= 0.0) || !(o->backtrack >= 0.0) || o->max_halvings < 0) return SCFT_E_ARG;
   SCFT_TRY(ensure_mixer(b, o->history + 1));
   const auto t0 = std::chrono::steady_clock::now();
   const int nb = static_cast<int>(b->buckets.size());
@@ -653,6 +661,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.r = b->d_r;
   a.rnorm = b->d_rnorm;
   a.phit = b->d_phit;
+  a.x = b->d_x;
   a.wcold = b->has_wcold ? b->d_wcold : nullptr;
   a.hw = b->d_hw;
   a.hr = b->d_hr;
@@ -671,10 +680,13 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.reg = o->regularization;
   a.floor = o->floor;
   a.fallback_thresh = 0.0;
+  a.precond = o->precond;
+  a.backtrack = o->backtrack;
+  a.max_halvings = o->max_halvings;
   // fresh mixer state (a small kernel-free host copy of zeros is not needed:
   // the per-instance state array is rewritten from a pinned staging copy)
   if (static_cast<int>(b->ms0.size()) != b->n_inst)
-    b->ms0.assign(b->n_inst, MixState{0, 0, 0, 0, INFINITY});
+    b->ms0.assign(b->n_inst, MixState{0, 0, 0, 0, INFINITY, INFINITY, 0, 0});
   HIPCHK(hipMemcpyAsync(b->d_ms, b->ms0.data(), sizeof(MixState) * b->n_inst,
                         hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));

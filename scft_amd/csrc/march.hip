@@ -207,7 +207,8 @@ __device__ __forceinline__ void assemble_row(const double* __restrict__ x,
 
 template <int K, int W>
 __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0,
-                          double* __restrict__ slab, MarchShared<W>& sh, const int tid) {
+                          double* __restrict__ slab, MarchShared<W>& sh,
+                          double* __restrict__ pw_lds, const int tid) {
   constexpr int P = 64 * W;
   const int lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -221,6 +222,12 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   const double* __restrict__ pw = a.pw + d.po;
   const int cmat = __builtin_amdgcn_readfirstlane(d.cmat);
   const int u0 = tid * K;
+
+  // pairing weights of this N into LDS (the previous instance's reads of the
+  // table ended at its residual barrier / in program order for one wave)
+  wg_barrier();
+  for (int i = tid; i <= (N >> 1); i += P) pw_lds[i] = pw[i];
+  wg_barrier();
 
   // ---------------------------------------------------------------- assembly
   double av[K], bv[K], cv[K], ra[K], rc[K];
@@ -369,18 +376,19 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   }
 
   // ------------------------------------------------------------------ march
-  double q[K], acc[K], qo[K];
+  double q[K], acc[K];
 #pragma unroll
   for (int t = 0; t < K; t++) {
     q[t] = (u0 + t < n) ? 1.0 : 0.0;
     acc[t] = 0.0;
-    qo[t] = 0.0;
   }
   double ql = (u0 - 1 >= 0 && u0 - 1 < n) ? 1.0 : 0.0;
   double qr = (u0 + K < n) ? 1.0 : 0.0;
   const int half = N >> 1;
 
-  for (int j = 1; j <= N; j++) {
+  // One implicit-Euler step q <- D^{-1} (A q) for the whole instance; ql / qr
+  // become the new values of the unknowns left of q[0] and right of q[K-1].
+  auto advance = [&]() __attribute__((always_inline)) {
     double rp[K];
 #pragma unroll
     for (int t = 0; t < K; t++) {
@@ -450,33 +458,53 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     }
     ql = zin - gl * q[0];
     qr = xin;
+  };
 
-    // ------------------------------------------- slabs and the density sum
-    if (j < half) {
-      double* dst = slab + (long long)(j - 1) * (P * K) + tid;
+  // Slab s (q after step s, s = 0 .. N/2 - 1; slab 0 is the initial
+  // condition) sits at s * P * K + t * P + tid: every lane of the workgroup,
+  // padding lanes included (they hold 0), so loads and stores need no mask.
+  // First half: nothing here waits on memory.
+  {
+    double* __restrict__ dst = slab + tid;
 #pragma unroll
-      for (int t = 0; t < K; t++)
-        if (u0 + t < n) dst[t * P] = q[t];
-    } else {
-      if (j == half) {
-        const double c0 = pw[0];
+    for (int t = 0; t < K; t++) dst[t * P] = q[t];
+    for (int j = 1; j < half; j++) {
+      advance();
+      dst += P * K;
 #pragma unroll
-        for (int t = 0; t < K; t++) acc[t] = c0 * q[t] * q[t];
-      } else {
-        const double cj = pw[j - half];
+      for (int t = 0; t < K; t++) dst[t * P] = q[t];
+    }
+  }
+  // step N/2 pairs with itself
+  advance();
+  {
+    const double c0 = pw_lds[0];
 #pragma unroll
-        for (int t = 0; t < K; t++) acc[t] = fma(cj * q[t], qo[t], acc[t]);
-      }
-      // prefetch slab N-j-1 for the next step (slab 0 is the initial ones)
-      const int sn = N - j - 1;
-      if (sn >= 1) {
-        const double* src = slab + (long long)(sn - 1) * (P * K) + tid;
+    for (int t = 0; t < K; t++) acc[t] = c0 * q[t] * q[t];
+  }
+
+  // Second half: step N/2 + i pairs with slab N/2 - i (i = 1 .. N/2).  Slabs
+  // come through a ring of PD register sets, each loaded PD steps before its
+  // use (indices below 0 reload slab 0 and are never used).  PD divides N/2
+  // (N is a power of two >= 16).
+  constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
+  double ring[PD][K];
+  auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
+    const double* __restrict__ src = slab + (long long)max(s, 0) * (P * K) + tid;
 #pragma unroll
-        for (int t = 0; t < K; t++) qo[t] = (u0 + t < n) ? src[t * P] : 0.0;
-      } else if (sn == 0) {
+    for (int t = 0; t < K; t++) dstv[t] = src[t * P];
+  };
 #pragma unroll
-        for (int t = 0; t < K; t++) qo[t] = (u0 + t < n) ? 1.0 : 0.0;
-      }
+  for (int e = 0; e < PD; e++) fetch(half - 1 - e, ring[e]);
+  for (int i0 = 1; i0 <= half; i0 += PD) {
+#pragma unroll
+    for (int e = 0; e < PD; e++) {
+      const int i = i0 + e;
+      advance();
+      const double cj = pw_lds[i];
+#pragma unroll
+      for (int t = 0; t < K; t++) acc[t] = fma(cj * q[t], ring[e][t], acc[t]);
+      fetch(half - i - PD, ring[e]);
     }
   }
 
@@ -523,13 +551,14 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
 template <int K, int W>
 __global__ void __launch_bounds__(64 * W) march_kernel(MarchArgs a) {
   __shared__ MarchShared<W> sh;
+  extern __shared__ double pw_lds[];  // half_max + 1 pairing weights
   const int tid = threadIdx.x;
   double* slab = a.slabs + (long long)blockIdx.x * a.slab_stride;
   for (int it = blockIdx.x; it < a.n_items; it += gridDim.x) {
     const int inst = __builtin_amdgcn_readfirstlane(a.items[it]);
     const unsigned st = __builtin_amdgcn_readfirstlane(a.status[inst]);
     if (st & ST_DONE_MASK) continue;
-    march_one<K, W>(a, inst, st, slab, sh, tid);
+    march_one<K, W>(a, inst, st, slab, sh, pw_lds, tid);
   }
 }
 
@@ -542,6 +571,10 @@ __global__ void __launch_bounds__(64 * W) march_kernel(MarchArgs a) {
   X(1, 4) X(2, 4) X(4, 4) X(8, 4) X(16, 4) \
   X(1, 8) X(2, 8) X(4, 8) X(8, 8)          \
   X(1, 16) X(2, 16)
+
+static size_t march_lds_bytes(int half_max) {
+  return static_cast<size_t>(half_max + 1) * sizeof(double);
+}
 
 static const void* march_fn(int K, int W) {
 #define X(k, w) \
@@ -556,7 +589,8 @@ bool march_variant_exists(int K, int W) { return march_fn(K, W) != nullptr; }
 hipError_t launch_march(int K, int W, const MarchArgs& a, int grid, hipStream_t s) {
 #define X(k, w)                                                                 \
   if (K == k && W == w) {                                                       \
-    hipLaunchKernelGGL((march_kernel<k, w>), dim3(grid), dim3(64 * w), 0, s, a); \
+    hipLaunchKernelGGL((march_kernel<k, w>), dim3(grid), dim3(64 * w),           \
+                       march_lds_bytes(a.half_max), s, a);                      \
     return hipGetLastError();                                                   \
   }
   SCFT_MARCH_VARIANTS(X)
@@ -564,14 +598,16 @@ hipError_t launch_march(int K, int W, const MarchArgs& a, int grid, hipStream_t 
   return hipErrorInvalidValue;
 }
 
-int march_max_grid(int K, int W) {
+int march_max_grid(int K, int W, int half_max) {
   const void* f = march_fn(K, W);
   if (!f) return 0;
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     return 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 64 * W, 0) != hipSuccess) return 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, 64 * W, march_lds_bytes(half_max)) !=
+      hipSuccess)
+    return 0;
   if (per < 1) per = 1;
   return per * cus;
 }

@@ -34,9 +34,10 @@ struct MarchArgs {
   double* rnorm;          // out: ||r||_inf per instance
   unsigned* status;       // per-instance status word
   double* slabs;          // q-slab scratch, one region per workgroup
-  long long slab_stride;  // doubles per workgroup region
+  long long slab_stride;  // doubles per workgroup region (N_max/2 slabs)
   const int* items;       // instance ids of this bucket, largest first
   int n_items;
+  int half_max;           // largest N/2 of the bucket (LDS weight table size - 1)
 };
 
 constexpr unsigned ST_CONVERGED = 1u;
@@ -44,6 +45,7 @@ constexpr unsigned ST_NAN = 2u;
 constexpr unsigned ST_PIVOT = 4u;
 constexpr unsigned ST_MAXITER = 8u;
 constexpr unsigned ST_FALLBACK = 16u;
+constexpr unsigned ST_FAILED = 32u;
 constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER;
 
 struct MixState {
@@ -52,6 +54,9 @@ struct MixState {
   int stored;    // valid ring entries
   int restarts;
   double best;   // best ||r||_inf so far
+  double eacc;   // ||r||_inf of the last accepted iterate (inf: none yet)
+  int halvings;  // consecutive backtracking halvings
+  int pad;
 };
 
 struct AndersonArgs {
@@ -60,6 +65,7 @@ struct AndersonArgs {
   const double* r;      // plain residual phi - phi_t from the march
   const double* rnorm;  // ||r||_inf from the march
   const double* phit;   // target on unknowns
+  const double* x;      // node coordinates (preconditioner stencil)
   const double* wcold;  // cold-start fields (nullable)
   double* hw;           // history of fields   (S * n per instance at ho)
   double* hr;           // history of residuals
@@ -73,13 +79,16 @@ struct AndersonArgs {
   int max_iter;
   double beta, tol, step_cap, restart_factor, reg, floor;
   double fallback_thresh;  // absolute threshold (<= 0: off), first sweep only
+  double precond;          // Laplacian preconditioner coefficient (0: off)
+  double backtrack;        // growth factor that triggers a halving (0: off)
+  int max_halvings;
 };
 
 // Launch one bucket.  Returns hipSuccess or the launch error.
 hipError_t launch_march(int K, int W, const MarchArgs& a, int grid,
                         hipStream_t s);
 // Largest grid of co-resident workgroups for a variant (persistent launch).
-int march_max_grid(int K, int W);
+int march_max_grid(int K, int W, int half_max);
 bool march_variant_exists(int K, int W);
 hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_reset(int* counters, int n, hipStream_t s);

@@ -63,3 +63,15 @@ def instance(m, L, N, bc=O.BC_DN, graded=None, seed=None, w=None):
 
 def unknowns(a, bc):
     return a[1:-1] if bc == O.BC_DD else a[1:]
+
+
+def workloads():
+    """scft_amd/workloads.py loaded on its own (pure numpy), so CPU tests do
+    not need the device library that `import scft_amd` loads."""
+    import importlib.util
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "scft_amd", "workloads.py")
+    spec = importlib.util.spec_from_file_location("scft_workloads", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod

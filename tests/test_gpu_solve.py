@@ -31,13 +31,15 @@ def _opts_pair(**kw):
     return o_dev, o_cpu
 
 
-@pytest.mark.parametrize("transform,beta,cap", [(0, 0.1, 0.5), (2, 1.0, 2.0)])
-def test_device_anderson_trajectory_matches_oracle(transform, beta, cap):
+@pytest.mark.parametrize("transform,beta,cap,precond",
+                         [(0, 0.1, 0.5, 0.0), (2, 1.0, 2.0, 0.0), (2, 1.0, 0.0, 0.5)])
+def test_device_anderson_trajectory_matches_oracle(transform, beta, cap, precond):
     S = _S()
     K = 4
     cfg = WL.config3(n_inst=3, seed=7, Ns=(256,))
     xs, Ns, pts, w0 = cfg["xs"], cfg["Ns"], cfg["pts"], cfg["w0"]
-    o_dev, o_cpu = _opts_pair(transform=transform, beta=beta, step_cap=cap, max_iter=K + 1)
+    o_dev, o_cpu = _opts_pair(transform=transform, beta=beta, step_cap=cap, max_iter=K + 1,
+                              precond=precond)
     o_cpu.max_iter = K
     b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
     out = b.solve([w[1:] for w in w0], o_dev)
@@ -106,4 +108,49 @@ def test_nan_field_is_flagged_not_propagated():
     assert out["status"][1] & S.ST_NAN
     assert out["status"][0] == 0 and out["status"][2] == 0
     assert np.isfinite(out["phi"][0]).all() and np.isfinite(out["phi"][2]).all()
+    b.close()
+
+
+def test_device_backtracking_matches_oracle():
+    """m = 2891, N = 256, preconditioned log mixing: evaluation 4 grows ||r||
+    100-fold, so with backtrack = 10 the fourth update is the halving
+    W_2 + (W_3 - W_2) / 2 on both sides; the instance keeps running."""
+    S = _S()
+    K = 4
+    cfg = WL.config3(n_inst=24, seed=11, Ns=(256,))
+    k = 5
+    x, pt, w0 = cfg["xs"][k], cfg["pts"][k], cfg["w0"][k]
+    o_dev, o_cpu = _opts_pair(transform=2, beta=1.0, step_cap=0.0, max_iter=K + 1,
+                              precond=0.5, backtrack=10.0)
+    o_cpu.max_iter = K
+    b = S.Batch([x], [256], [pt[1:]], bc=S.BC_DN)
+    out = b.solve([w0[1:]], o_dev)
+    ref = O.solve_anderson(x, pt, 256, w0, o_cpu, bc=O.BC_DN)
+    assert ref["restarts"] == 1
+    dw = np.abs(out["w"][0] - ref["w"][1:]).max()
+    print(f"|w_dev - w_cpu| after the halving = {dw:.2e}")
+    assert dw <= 1e-6 * max(1.0, np.abs(ref["w"]).max())
+    assert not (out["status"][0] & (S.ST_FAILED | S.ST_NAN))
+    b.close()
+
+
+def test_preconditioned_batch_solve_converges_fast():
+    """Mixed sizes m = 32 .. 4096, N = 512 / 1024, preconditioned log mixing
+    (a = 0.5, no step cap, backtracking on NaN only): every instance
+    converges within 60 evaluations and the CPU oracle confirms the
+    residual of every returned field."""
+    S = _S()
+    cfg = WL.config3(n_inst=48, seed=13, Ns=(512, 1024))
+    xs, Ns, pts, w0 = cfg["xs"], cfg["Ns"], cfg["pts"], cfg["w0"]
+    b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
+    opts = S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=200, step_cap=0.0,
+                           transform=S.MIX_LOG, precond=0.5, backtrack=1e30, check_every=4)
+    out = b.solve([w[1:] for w in w0], opts)
+    print("iterations:", sorted(int(v) for v in out["iters"]))
+    assert out["converged"] == len(xs), out["status"]
+    assert int(np.max(out["iters"])) <= 60
+    for k in range(len(xs)):
+        w = np.concatenate([[0.0], out["w"][k]])
+        r = O.evaluate(xs[k], w, pts[k], Ns[k], bc=O.BC_DN)["rnorm"]
+        assert r <= 1e-6 + 1e-10
     b.close()
