@@ -49,6 +49,22 @@ SOLVER_TEXT = ("device Anderson, history 10, log mixing preconditioned by I + 0.
                "beta 1.0, no step cap, NaN backtracking")
 
 
+def measured_traffic(kern, n_items, alg_bytes):
+    """HBM bytes of the dominant launch from the committed counter runs
+    (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when they were taken on this
+    same launch (variant, bucket, algorithmic bytes); otherwise None."""
+    path = os.path.join(ROOT, "profiles", "march_config3_4096_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("kernel") != kern or t.get("items") != n_items or \
+            abs(t.get("algorithmic_bytes", -1.0) - alg_bytes) > 1e-6 * alg_bytes:
+        return None
+    return round(float(t["traffic_bytes"]))
+
+
 def solver_opts(S, max_iter):
     return S.anderson_opts(max_iter=max_iter, transform=S.MIX_LOG, check_every=4,
                            progress_seconds=20.0, **SOLVER)
@@ -101,12 +117,29 @@ def main():
                     help="seconds of CPU-baseline work (per thread) to time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="only the dominant-launch measurement (for profiler runs)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="run a plain device copy of MIB MiB 5 times and exit (counter scale)")
     ap.add_argument("--tune-file", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): instances are "
               "independent, so a multi-GPU run is N copies of this one", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per
+        # dispatch calibrate the counters read for the march kernel
+        import torch
+        n = args.copy_calibration * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copied {args.copy_calibration} MiB 5 times")
+        return
 
     import scft_amd as S
     from scft_amd import workloads as WL
@@ -124,12 +157,19 @@ def main():
 
     # roofline of the dominant march launch (HIP events on the launch stream)
     buckets = b.time_buckets(w0, reps=3)
-    dom = max(buckets, key=lambda t: t[1])
-    n_items, ms, by = dom
+    tuning = b.tuning()  # same bucket order as time_buckets
+    di = max(range(len(buckets)), key=lambda i: buckets[i][1])
+    n_items, ms, by = buckets[di]
+    kern = f"march_kernel<{tuning[di][1]}, {tuning[di][2]}>"
     achieved = by / (ms * 1e-3) / 1e9
     sweep_ms = sum(t[1] for t in buckets)
     log(f"bench: buckets (items, ms, GB) = {[(i, round(m, 3), round(bb / 1e9, 3)) for i, m, bb in buckets]}")
     log(f"bench: one sweep {sweep_ms:.2f} ms, dominant launch {ms:.3f} ms, {achieved:.1f} GB/s")
+    if args.roofline_only:
+        print(json.dumps({"roofline": {"achieved": round(achieved, 1), "ms": ms, "bytes": by,
+                                       "items": n_items, "kernel": kern}}), flush=True)
+        b.close()
+        return
 
     opts = solver_opts(S, args.max_iter)
     for i in range(args.warmup):
@@ -179,8 +219,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": f"march bucket with {n_items} instances, {ms:.3f} ms per launch",
+            "traffic": measured_traffic(kern, n_items, by),
+            "kernel": f"{kern}: bucket of {n_items} instances, {ms:.3f} ms per launch",
         },
     }
     if not args.no_cpu:

@@ -473,7 +473,7 @@ extern "C" int scft_batch_time_buckets(scft_batch* b, const double* w, int reps,
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));
   HIPCHK(hipEventCreate(&e1));
-  for (int i = nb - 1; i >= 0; i--) {
+  for (int i = 0; i < nb; i++) {  // the order of scft_batch_tuning
     const Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
     const MarchArgs a = march_args(b, bk, i);
