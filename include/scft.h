@@ -192,7 +192,7 @@ int scft_batch_download(scft_batch *b, double *w, int *iters, unsigned *status,
  * device field with HIP events on the launch stream. */
 int scft_batch_time_eval(scft_batch *b, const double *w, int reps,
                          double *ms_per_eval, double *march_ms);
-/* Per size bucket (largest first): average duration of one persistent march
+/* Per size bucket (in the order of scft_batch_tuning): average duration of one persistent march
  * launch over `reps` (HIP events on the launch stream), its algorithmic HBM
  * bytes (q-slabs written and read back plus O(n) I/O) and instance count. */
 int scft_batch_time_buckets(scft_batch *b, const double *w, int reps,
