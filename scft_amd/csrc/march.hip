@@ -158,51 +158,55 @@ __device__ __forceinline__ dd shfl_up_dd(dd v, int s) {
 // every row the same way and move the diagonal-dominance margin
 // b - |a| - |c| ~ h by eps * dt/h^2 relative per step, which adds up over the
 // march (~1e-10 after a few hundred steps when dt/h^2 ~ 1e3).
-__device__ __forceinline__ void assemble_row(const double* __restrict__ x,
-                                             const double* __restrict__ w, int n, int M, int u,
-                                             double dt, int cmat, double wall0, double wallL,
-                                             double& av, double& bv, double& cv, double& ra,
-                                             double& rc) {
+struct Row {
+  double a, b, c;  // D = A + dt (B + C): sub-, main and super-diagonal
+  double ra, rc;   // A q = ra (q_{i-1} + 2 q_i) + rc (2 q_i + q_{i+1})
+};
+
+// Returned by value with every field assigned once (selects, no stores
+// through references): conditional stores to two different outputs get
+// merged into one store through a pointer phi, which keeps the row arrays
+// out of registers.
+__device__ __forceinline__ Row assemble_row(const double* __restrict__ x,
+                                            const double* __restrict__ w, int n, int M, int u,
+                                            double dt, int cmat, double wall0, double wallL) {
 #pragma clang fp contract(off)
-  if (u < n) {
-    const int i = u + 1;
-    const double xl = x[i - 1], xc = x[i];
-    const bool has_r = (i + 1 < M);
-    const double xr = has_r ? x[i + 1] : xc;
-    const double hl = xc - xl, hr = xr - xc;
-    const double wi = w[u];
-    const double Al = hl / 6.0, Ar = hr / 6.0, Ad = hl / 3.0 + hr / 3.0;
-    const double Bl = -1.0 / hl;
-    const double Br = has_r ? -1.0 / hr : 0.0;
-    const double Bd = 1.0 / hl + (has_r ? 1.0 / hr : 0.0);
-    double Cl, Cd, Cr;
-    if (cmat == 0) {
-      Cl = wi * Al;
-      Cd = wi * Ad;
-      Cr = wi * Ar;
-    } else {
-      const double wl = (u > 0) ? w[u - 1] : wall0;
-      const double wr = (u + 1 < n) ? w[u + 1] : wallL;
-      Cl = hl * (wl + wi) / 12.0;
-      Cd = hl * (wl + 3.0 * wi) / 12.0 + hr * (3.0 * wi + wr) / 12.0;
-      Cr = hr * (wi + wr) / 12.0;
-    }
-    av = Al + dt * (Bl + Cl);
-    bv = Ad + dt * (Bd + Cd);
-    cv = Ar + dt * (Br + Cr);
-    // right-hand side A q split by element: (h_l/6)(q_{i-1} + 2 q_i) +
-    // (h_r/6)(2 q_i + q_{i+1}); wall neighbours are held at q = 0
-    ra = Al;
-    rc = Ar;
-    if (u == 0) av = 0.0;      // left neighbour: the Dirichlet wall
-    if (u == n - 1) cv = 0.0;  // right neighbour: a wall (DD) or none (DN)
-  } else {  // padding: decoupled identity rows, q stays 0
-    av = 0.0;
-    bv = 1.0;
-    cv = 0.0;
-    ra = 0.0;
-    rc = 0.0;
+  const bool live = u < n;
+  const int i = live ? u + 1 : 1;  // padding rows read node 1 and are discarded
+  const double xl = x[i - 1], xc = x[i];
+  const bool has_r = (i + 1 < M);
+  const double xr = has_r ? x[i + 1] : xc;
+  const double hl = xc - xl, hr = xr - xc;
+  const double wi = live ? w[live ? u : 0] : 0.0;  // index clamped: no load past the field
+  const double Al = hl / 6.0, Ar = hr / 6.0, Ad = hl / 3.0 + hr / 3.0;
+  const double Bl = -1.0 / hl;
+  const double Br = has_r ? -1.0 / hr : 0.0;
+  const double Bd = 1.0 / hl + (has_r ? 1.0 / hr : 0.0);
+  double Cl, Cd, Cr;
+  if (cmat == 0) {
+    Cl = wi * Al;
+    Cd = wi * Ad;
+    Cr = wi * Ar;
+  } else {
+    const double wl = (u > 0 && live) ? w[u - 1] : wall0;
+    const double wr = (u + 1 < n) ? w[u + 1] : wallL;
+    Cl = hl * (wl + wi) / 12.0;
+    Cd = hl * (wl + 3.0 * wi) / 12.0 + hr * (3.0 * wi + wr) / 12.0;
+    Cr = hr * (wi + wr) / 12.0;
   }
+  const double av = Al + dt * (Bl + Cl);
+  const double bv = Ad + dt * (Bd + Cd);
+  const double cv = Ar + dt * (Br + Cr);
+  // left neighbour of unknown 0 is the Dirichlet wall; right neighbour of
+  // unknown n-1 is a wall (DD) or none (DN); padding rows are decoupled
+  // identity rows with a zero right-hand side (q stays 0 there)
+  Row r;
+  r.a = (live && u > 0) ? av : 0.0;
+  r.c = (live && u < n - 1) ? cv : 0.0;
+  r.b = live ? bv : 1.0;
+  r.ra = live ? Al : 0.0;
+  r.rc = live ? Ar : 0.0;
+  return r;
 }
 
 template <int K, int W>
@@ -232,9 +236,14 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   // ---------------------------------------------------------------- assembly
   double av[K], bv[K], cv[K], ra[K], rc[K];
 #pragma unroll
-  for (int t = 0; t < K; t++)
-    assemble_row(x, w, n, M, u0 + t, dt, cmat, d.wall0, d.wallL, av[t], bv[t], cv[t], ra[t],
-                 rc[t]);
+  for (int t = 0; t < K; t++) {
+    const Row rw = assemble_row(x, w, n, M, u0 + t, dt, cmat, d.wall0, d.wallL);
+    av[t] = rw.a;
+    bv[t] = rw.b;
+    cv[t] = rw.c;
+    ra[t] = rw.ra;
+    rc[t] = rw.rc;
+  }
 
   // ------------------------------------------------------------ factor scan
   double cprev = __shfl_up(cv[K - 1], 1);
