@@ -49,17 +49,25 @@ SOLVER_TEXT = ("device Anderson, history 10, log mixing preconditioned by I + 0.
                "beta 1.0, no step cap, NaN backtracking")
 
 
+def march_source_sha256():
+    import hashlib
+    with open(os.path.join(ROOT, "scft_amd", "csrc", "march.hip"), "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
 def measured_traffic(kern, n_items, alg_bytes):
     """HBM bytes of the dominant launch from the committed counter runs
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when they were taken on this
-    same launch (variant, bucket, algorithmic bytes); otherwise None."""
+    same launch (kernel source, variant, bucket, algorithmic bytes);
+    otherwise None."""
     path = os.path.join(ROOT, "profiles", "march_config3_4096_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
     except (OSError, ValueError):
         return None
-    if t.get("kernel") != kern or t.get("items") != n_items or \
+    if t.get("source_sha256") != march_source_sha256() or t.get("kernel") != kern or \
+            t.get("items") != n_items or \
             abs(t.get("algorithmic_bytes", -1.0) - alg_bytes) > 1e-6 * alg_bytes:
         return None
     return round(float(t["traffic_bytes"]))

@@ -470,18 +470,26 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   };
 
   // Slab s (q after step s, s = 0 .. N/2 - 1; slab 0 is the initial
-  // condition) sits at s * P * K + t * P + tid: every lane of the workgroup,
-  // padding lanes included (they hold 0), so loads and stores need no mask.
+  // condition) sits at s * P * K + t * P + tid.  Lanes whose rows are all
+  // padding (u0 >= n) store nothing (one exec mask for the whole loop) and
+  // load from lane 0's address, so a wave's dead lanes share one cache line;
+  // what they load only ever multiplies their q = 0.
   // First half: nothing here waits on memory.
+  const bool lane_live = u0 < n;
+  const int lane_src = lane_live ? tid : 0;
   {
     double* __restrict__ dst = slab + tid;
+    if (lane_live) {
 #pragma unroll
-    for (int t = 0; t < K; t++) dst[t * P] = q[t];
+      for (int t = 0; t < K; t++) dst[t * P] = q[t];
+    }
     for (int j = 1; j < half; j++) {
       advance();
       dst += P * K;
+      if (lane_live) {
 #pragma unroll
-      for (int t = 0; t < K; t++) dst[t * P] = q[t];
+        for (int t = 0; t < K; t++) dst[t * P] = q[t];
+      }
     }
   }
   // step N/2 pairs with itself
@@ -499,7 +507,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
   double ring[PD][K];
   auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
-    const double* __restrict__ src = slab + (long long)max(s, 0) * (P * K) + tid;
+    const double* __restrict__ src = slab + (long long)max(s, 0) * (P * K) + lane_src;
 #pragma unroll
     for (int t = 0; t < K; t++) dstv[t] = src[t * P];
   };
@@ -557,13 +565,24 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   }
 }
 
+// Persistent workgroups take items (sorted by cost, largest first) from a
+// per-bucket counter: greedy list scheduling, which also rebalances as
+// instances converge.  Thread 0 takes the next index and broadcasts it
+// through LDS; the exit test is on that workgroup-uniform value, so every
+// wave leaves the loop together once the counter passes n_items.
 template <int K, int W>
 __global__ void __launch_bounds__(64 * W) march_kernel(MarchArgs a) {
   __shared__ MarchShared<W> sh;
+  __shared__ int s_next;
   extern __shared__ double pw_lds[];  // half_max + 1 pairing weights
   const int tid = threadIdx.x;
   double* slab = a.slabs + (long long)blockIdx.x * a.slab_stride;
-  for (int it = blockIdx.x; it < a.n_items; it += gridDim.x) {
+  for (;;) {
+    if (tid == 0) s_next = atomicAdd(a.queue, 1);
+    wg_barrier();
+    const int it = __builtin_amdgcn_readfirstlane(s_next);
+    wg_barrier();  // everyone has read s_next before thread 0 overwrites it
+    if (it >= a.n_items) break;
     const int inst = __builtin_amdgcn_readfirstlane(a.items[it]);
     const unsigned st = __builtin_amdgcn_readfirstlane(a.status[inst]);
     if (st & ST_DONE_MASK) continue;

@@ -36,6 +36,7 @@ struct MarchArgs {
   double* slabs;          // q-slab scratch, one region per workgroup
   long long slab_stride;  // doubles per workgroup region (N_max/2 slabs)
   const int* items;       // instance ids of this bucket, largest first
+  int* queue;             // next item to take (zeroed before every launch)
   int n_items;
   int half_max;           // largest N/2 of the bucket (LDS weight table size - 1)
 };
