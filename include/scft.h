@@ -205,6 +205,45 @@ int scft_batch_autotune(scft_batch *b, int reps);
 int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
                       int *K, int *W, int max_buckets);
 
+/* ---- warm-start MLP (no reference counterpart) -------------------------
+ * Dense FP32 network trained and evaluated on the device by hand-written
+ * kernels (matrix-core GEMMs with fused bias / tanh / tanh' epilogues, MSE,
+ * Adam): tanh hidden layers, linear output.  dims[0] inputs ... dims[n_layers]
+ * outputs; parameters packed per layer as W (dims[l+1] x dims[l], row-major)
+ * then b (dims[l+1]); weights start Glorot-uniform from splitmix64(seed),
+ * biases at 0. */
+typedef struct scft_mlp scft_mlp;
+int scft_mlp_create(scft_mlp **out, int n_layers, const int *dims,
+                    unsigned long long seed);
+int scft_mlp_destroy(scft_mlp *m);
+int scft_mlp_param_count(const scft_mlp *m, long long *count);
+int scft_mlp_get_params(const scft_mlp *m, float *params);
+int scft_mlp_set_params(scft_mlp *m, const float *params); /* resets Adam */
+/* x' = (x - x_mean) / x_std on inputs and y = y' y_std + y_mean on outputs,
+ * applied by scft_mlp_forward and scft_mlp_train; all NULL clears them. */
+int scft_mlp_set_normalisation(scft_mlp *m, const float *x_mean,
+                               const float *x_std, const float *y_mean,
+                               const float *y_std);
+int scft_mlp_forward(scft_mlp *m, int batch, const float *X, float *Y);
+/* loss = mean((Y - T)^2) over batch x outputs and its gradient (grad
+ * nullable), on raw rows (no normalisation): for gradient checks. */
+int scft_mlp_loss_grad(scft_mlp *m, int batch, const float *X, const float *T,
+                       double *loss, float *grad);
+int scft_mlp_adam_step(scft_mlp *m, float lr); /* b1 0.9, b2 0.999, eps 1e-8 */
+/* Mini-batch Adam, reshuffled each epoch from splitmix64(seed); mean loss of
+ * each epoch (normalised units) into epoch_loss (nullable). */
+int scft_mlp_train(scft_mlp *m, int n_samples, const float *X, const float *Y,
+                   int epochs, int batch, float lr, unsigned long long seed,
+                   double *epoch_loss);
+
+/* Cubic spline through (x_src, y_src) (x increasing, n_src >= 4) with the
+ * not-a-knot end conditions of DEALII_SCFT/src/spline_chen.c:13-107 (the
+ * reference's m == NULL case), evaluated at x_dst; points outside the table
+ * use the end intervals' cubics, as there.  O(n) banded solve instead of the
+ * reference's dense gaussj. */
+int scft_resample_spline(int n_src, const double *x_src, const double *y_src,
+                         int n_dst, const double *x_dst, double *y_dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -107,6 +107,21 @@ def _load():
     L.scft_batch_autotune.argtypes = [C.c_void_p, C.c_int]
     L.scft_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), _ip, _ip, _ip, C.c_int]
     L.scft_debug_slabs.argtypes = [C.c_void_p, C.c_int, _dp]
+    _fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    L.scft_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _ip, C.c_ulonglong]
+    L.scft_mlp_destroy.argtypes = [C.c_void_p]
+    L.scft_mlp_param_count.argtypes = [C.c_void_p, C.POINTER(C.c_longlong)]
+    L.scft_mlp_get_params.argtypes = [C.c_void_p, _fp]
+    L.scft_mlp_set_params.argtypes = [C.c_void_p, _fp]
+    L.scft_mlp_set_normalisation.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p]
+    L.scft_mlp_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+    L.scft_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.POINTER(C.c_double),
+                                     C.c_void_p]
+    L.scft_mlp_adam_step.argtypes = [C.c_void_p, C.c_float]
+    L.scft_mlp_train.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_float,
+                                 C.c_ulonglong, C.c_void_p]
+    L.scft_resample_spline.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
     return L
 
 
@@ -302,3 +317,87 @@ class Batch:
         out = np.empty((N // 2 - 1) * n)
         check(lib.scft_debug_slabs(self.h, inst, out), "scft_debug_slabs")
         return out.reshape(N // 2 - 1, n)
+
+
+# ---------------------------------------------------------------- resampling
+def resample_spline(x_src, y_src, x_dst):
+    """Not-a-knot cubic spline through (x_src, y_src) evaluated at x_dst
+    (DEALII_SCFT/src/spline_chen.c:13-107 semantics)."""
+    xs, ys, xd = _f64(x_src), _f64(y_src), _f64(x_dst)
+    out = np.empty(len(xd))
+    check(lib.scft_resample_spline(len(xs), xs, ys, len(xd), xd, out), "scft_resample_spline")
+    return out
+
+
+# ---------------------------------------------------------------- MLP
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """Dense FP32 network on the device (tanh hidden layers, linear output),
+    trained with the library's own forward / backward / Adam kernels."""
+
+    def __init__(self, dims, seed=1):
+        self.dims = [int(d) for d in dims]
+        h = C.c_void_p()
+        d = np.ascontiguousarray(self.dims, dtype=np.int32)
+        check(lib.scft_mlp_create(C.byref(h), len(self.dims) - 1, d, seed), "scft_mlp_create")
+        self.h = h
+        n = C.c_longlong()
+        check(lib.scft_mlp_param_count(self.h, C.byref(n)), "scft_mlp_param_count")
+        self.n_params = n.value
+
+    def close(self):
+        if self.h:
+            lib.scft_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.empty(self.n_params, dtype=np.float32)
+        check(lib.scft_mlp_get_params(self.h, p), "scft_mlp_get_params")
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.n_params
+        check(lib.scft_mlp_set_params(self.h, p), "scft_mlp_set_params")
+
+    def set_normalisation(self, x_mean=None, x_std=None, y_mean=None, y_std=None):
+        if x_mean is None:
+            check(lib.scft_mlp_set_normalisation(self.h, None, None, None, None), "norm")
+            return
+        self._norm = [_f32(v) for v in (x_mean, x_std, y_mean, y_std)]
+        check(lib.scft_mlp_set_normalisation(self.h, *[v.ctypes.data for v in self._norm]),
+              "scft_mlp_set_normalisation")
+
+    def forward(self, X):
+        X = _f32(X)
+        Y = np.empty((X.shape[0], self.dims[-1]), dtype=np.float32)
+        check(lib.scft_mlp_forward(self.h, X.shape[0], X, Y), "scft_mlp_forward")
+        return Y
+
+    def loss_grad(self, X, T, want_grad=True):
+        X, T = _f32(X), _f32(T)
+        loss = C.c_double()
+        g = np.empty(self.n_params, dtype=np.float32) if want_grad else None
+        check(lib.scft_mlp_loss_grad(self.h, X.shape[0], X, T, C.byref(loss),
+                                     g.ctypes.data if g is not None else None),
+              "scft_mlp_loss_grad")
+        return loss.value, g
+
+    def adam_step(self, lr=1e-3):
+        check(lib.scft_mlp_adam_step(self.h, lr), "scft_mlp_adam_step")
+
+    def train(self, X, Y, epochs=50, batch=256, lr=1e-3, seed=1):
+        X, Y = _f32(X), _f32(Y)
+        hist = np.zeros(epochs)
+        check(lib.scft_mlp_train(self.h, X.shape[0], X, Y, epochs, batch, lr, seed,
+                                 hist.ctypes.data), "scft_mlp_train")
+        return hist

@@ -986,3 +986,14 @@ extern "C" void scft_adm_chen_f(int n, double* in, double* out) {
   }
   for (int i = 0; i < n; i++) out[i] = -out[i];
 }
+
+// ------------------------------------------------------------ resampling
+extern "C" int scft_resample_spline(int n_src, const double* x_src, const double* y_src,
+                                    int n_dst, const double* x_dst, double* y_dst) {
+  if (n_src < 4 || n_dst < 0 || !x_src || !y_src || (n_dst > 0 && (!x_dst || !y_dst)))
+    return SCFT_E_ARG;
+  std::vector<double> M;
+  if (!scft::spline_not_a_knot(n_src, x_src, y_src, M)) return SCFT_E_ARG;
+  for (int i = 0; i < n_dst; i++) y_dst[i] = scft::spline_eval(n_src, x_src, y_src, M.data(), x_dst[i]);
+  return SCFT_OK;
+}

@@ -85,3 +85,72 @@ void mesh_uniform(int M, double L, double* x) {
 }
 
 }  // namespace scft
+
+namespace scft {
+
+// Second derivatives M of the not-a-knot cubic spline through (x, y), the
+// interpolant of DEALII_SCFT/src/spline_chen.c:13-107 (m == NULL).  The
+// not-a-knot rows (third derivative continuous across x_1 and x_{n-2}) give
+//   M_0     = (1 + h_0/h_1) M_1 - (h_0/h_1) M_2,
+//   M_{n-1} = (1 + h_{n-2}/h_{n-3}) M_{n-2} - (h_{n-2}/h_{n-3}) M_{n-3};
+// substituted into the first and last interior rows they leave a
+// diagonally dominant tridiagonal system in M_1 .. M_{n-2}, solved by
+// Thomas.  Same solution as the reference's dense Gauss-Jordan solve.
+bool spline_not_a_knot(int n, const double* x, const double* y, std::vector<double>& M) {
+  if (n < 4) return false;
+  for (int i = 0; i + 1 < n; i++)
+    if (!(x[i + 1] > x[i])) return false;
+  const int k = n - 2;  // unknowns M_1 .. M_{n-2}
+  std::vector<double> lo(k), di(k), up(k), rhs(k);
+  for (int i = 1; i <= n - 2; i++) {
+    const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+    lo[i - 1] = hl / 6.0;
+    di[i - 1] = (hl + hr) / 3.0;
+    up[i - 1] = hr / 6.0;
+    rhs[i - 1] = (y[i + 1] - y[i]) / hr - (y[i] - y[i - 1]) / hl;
+  }
+  const double h0 = x[1] - x[0], h1 = x[2] - x[1];
+  const double ha = x[n - 2] - x[n - 3], hb = x[n - 1] - x[n - 2];
+  // row of M_1: lo * M_0 folded in
+  di[0] += lo[0] * (1.0 + h0 / h1);
+  if (k > 1) up[0] -= lo[0] * (h0 / h1);
+  lo[0] = 0.0;
+  // row of M_{n-2}: up * M_{n-1} folded in
+  di[k - 1] += up[k - 1] * (1.0 + hb / ha);
+  if (k > 1) lo[k - 1] -= up[k - 1] * (hb / ha);
+  up[k - 1] = 0.0;
+  // Thomas
+  for (int i = 1; i < k; i++) {
+    const double w = lo[i] / di[i - 1];
+    di[i] -= w * up[i - 1];
+    rhs[i] -= w * rhs[i - 1];
+  }
+  std::vector<double> m(k);
+  m[k - 1] = rhs[k - 1] / di[k - 1];
+  for (int i = k - 2; i >= 0; i--) m[i] = (rhs[i] - up[i] * m[i + 1]) / di[i];
+  M.assign(n, 0.0);
+  for (int i = 0; i < k; i++) M[i + 1] = m[i];
+  M[0] = (1.0 + h0 / h1) * M[1] - (h0 / h1) * M[2];
+  M[n - 1] = (1.0 + hb / ha) * M[n - 2] - (hb / ha) * M[n - 3];
+  return true;
+}
+
+// Value at xp of the spline (x, y, M): bisection for the bracketing
+// interval (the end intervals outside the table) and the cubic in the
+// form of spline_chen.c:88-100.
+double spline_eval(int n, const double* x, const double* y, const double* M, double xp) {
+  int klo = 0, khi = n - 1;
+  while (khi - klo > 1) {
+    const int k = (khi + klo) >> 1;
+    if (x[k] > xp)
+      khi = k;
+    else
+      klo = k;
+  }
+  const double h = x[khi] - x[klo];
+  const double a = (x[khi] - xp) / h, b = (xp - x[klo]) / h;
+  return a * y[klo] + b * y[khi] +
+         ((a * a * a - a) * M[klo] + (b * b * b - b) * M[khi]) * (h * h) / 6.0;
+}
+
+}  // namespace scft

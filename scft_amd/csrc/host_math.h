@@ -8,4 +8,6 @@ bool pairing_weights(int N, std::vector<double>& pw);
 double exp_profile(double x, double tau);
 void target_position(int M, const double* x, double tau, bool mirror, double* phi_t);
 void mesh_uniform(int M, double L, double* x);
+bool spline_not_a_knot(int n, const double* x, const double* y, std::vector<double>& M);
+double spline_eval(int n, const double* x, const double* y, const double* M, double xp);
 }  // namespace scft
