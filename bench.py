@@ -7,17 +7,29 @@ log-uniform in [32, 4096], contour steps N uniform over {512, 1024, 2048,
 4096}, L uniform in [2, 8], w0 = 0.1 exp(-x/0.2) + N(0, 1e-3) per instance
 seed; Dirichlet wall at x = 0, zero-flux end at L, EXP wall target.
 
-A step is one complete batched solve from the cold-start fields: every
-instance is driven by the device Anderson mixer until its residual is below
-1e-6, with the fields already resident in HBM when the timed region starts.
-`value` = instances solved per second over the K timed steps.
+A step is one complete batched solve of the 4096 instances with the fields
+already resident in HBM: every instance is driven by the device Anderson
+mixer until its residual is below 1e-6.  Two legs are timed, K steps each:
+
+  cold   every instance starts from w0 (the reference's initial guess)
+  warm   every instance starts from the prediction of the warm-start MLP,
+         trained (untimed setup) on 20000 fields that the batched solver
+         itself converged for instances drawn like the bench batch from
+         disjoint seeds; the timed step includes the prediction (network
+         forward on the device, spline resampling onto every mesh); an
+         instance whose first residual exceeds 10x the batch median
+         restarts from w0
+
+`value` = warm-start instances solved per second (cold with --no-warm);
+config.cold_start holds the cold leg and config.warm_over_cold the ratio.
 
 Also reported:
   roofline      the dominant march launch (largest per-bucket time) of one
                 evaluation sweep: algorithmic bytes (q-slabs written and read
                 back, O(n) I/O) over its HIP-event duration, against 8 TB/s
-  cpu_baseline  the CPU oracle (same mixer, same tolerance) on a bounded
-                sample of the same instances, OpenMP over the host cores
+  cpu_baseline  the CPU oracle (same mixer, same tolerance, cold start) on a
+                bounded sample of the same instances, threads over the host
+                cores
 """
 from __future__ import annotations
 
@@ -73,6 +85,74 @@ def measured_traffic(kern, n_items, alg_bytes):
     return round(float(t["traffic_bytes"]))
 
 
+def warm_start_leg(S, WL, b, cfg, w0, args):
+    """Train the warm-start network on fields the batched solver converges
+    for `args.train_inst` instances drawn like the bench batch from disjoint
+    seeds (setup, untimed), then time K warm-started solves of the bench
+    batch.  A timed step covers the prediction (network forward on the
+    device, spline resampling onto every mesh) and the device-resident solve;
+    an instance whose first residual exceeds 10x the batch median restarts
+    from w0."""
+    from scft_amd import warmstart as WS
+    t0 = time.perf_counter()
+    Xs, Ys = [], []
+    opts = solver_opts(S, args.max_iter)
+    left, part = args.train_inst, 0
+    while left > 0:
+        n = min(4096, left)
+        tcfg = WL.config3(n_inst=n, seed=7000 + 101 * part)  # seeds disjoint from the bench
+        out = WS.solve(S, tcfg, opts)
+        conv = (out["status"] & S.ST_CONVERGED) != 0
+        X, Y = WS.dataset(S, tcfg, out["w_nodes"], conv)
+        Xs.append(X)
+        Ys.append(Y)
+        left -= n
+        part += 1
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
+    t_data = time.perf_counter() - t0
+    ws = WS.WarmStart(S, seed=11)
+    t0 = time.perf_counter()
+    hist = ws.fit(X, Y, epochs=50, batch=256, lr=1e-3, seed=12)
+    t_train = time.perf_counter() - t0
+    log(f"bench: warm start: {len(X)} training fields in {t_data:.1f} s, 50 epochs in "
+        f"{t_train:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    prep = WS.Prepared(S, cfg)
+    wopts = solver_opts(S, args.max_iter)
+    wopts.fallback_factor = 10.0
+    w0_flat = b.join(w0)
+
+    def step():
+        t = time.perf_counter()
+        wi = ws.predict_unknowns(prep)
+        t_pred = time.perf_counter() - t
+        b.upload(wi, w0_flat)
+        r = b.solve_resident(wopts)
+        return t_pred, r
+
+    for i in range(args.warmup):
+        t_pred, r = step()
+        log(f"bench: warm warmup {i}: {t_pred + r['seconds']:.3f} s")
+    total, res, tp = 0.0, None, 0.0
+    for i in range(args.steps):
+        t_pred, r = step()
+        total += t_pred + r["seconds"]
+        tp += t_pred
+        res = b.download()
+        fb = int(np.sum((res["status"] & S.ST_FALLBACK) != 0))
+        log(f"bench: warm step {i}: {t_pred:.3f} s predict + {r['seconds']:.3f} s solve, "
+            f"{r['sweeps']} sweeps, {res['converged']}/{len(cfg['xs'])} converged, mean iters "
+            f"{float(np.mean(res['iters'])):.1f}, max {int(np.max(res['iters']))}, {fb} fallbacks")
+    ws.close()
+    info = {
+        "network": "131 -> 4 x 256 tanh -> 136, FP32, device forward/backward/Adam kernels",
+        "training_fields": int(len(X)), "epochs": 50, "batch": 256, "lr": 1e-3,
+        "final_loss": round(float(hist[-1]), 6), "train_seconds": round(t_train, 2),
+        "predict_ms_per_step": round(1e3 * tp / args.steps, 2),
+        "fallbacks": int(np.sum((res["status"] & S.ST_FALLBACK) != 0)),
+    }
+    return dict(res=res, total=total, value=res["converged"] * args.steps / total, info=info)
+
+
 def solver_opts(S, max_iter):
     return S.anderson_opts(max_iter=max_iter, transform=S.MIX_LOG, check_every=4,
                            progress_seconds=20.0, **SOLVER)
@@ -125,6 +205,9 @@ def main():
                     help="seconds of CPU-baseline work (per thread) to time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-warm", action="store_true", help="cold start only")
+    ap.add_argument("--train-inst", type=int, default=20000,
+                    help="converged fields the warm-start network is trained on")
     ap.add_argument("--roofline-only", action="store_true",
                     help="only the dominant-launch measurement (for profiler runs)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -179,22 +262,31 @@ def main():
         b.close()
         return
 
+    # ---- cold start: every instance from w0 (the reference's initial guess)
     opts = solver_opts(S, args.max_iter)
     for i in range(args.warmup):
         b.upload(w0)
         r = b.solve_resident(opts)
-        log(f"bench: warmup {i}: {r['seconds']:.2f} s, {r['sweeps']} sweeps")
-    total = 0.0
-    res = None
+        log(f"bench: cold warmup {i}: {r['seconds']:.2f} s, {r['sweeps']} sweeps")
+    total_cold = 0.0
+    res_cold = None
     for i in range(args.steps):
         b.upload(w0)
         r = b.solve_resident(opts)
-        total += r["seconds"]
-        res = b.download()
-        log(f"bench: step {i}: {r['seconds']:.2f} s, {r['sweeps']} sweeps, "
-            f"{res['converged']}/{args.n_inst} converged, mean iters "
-            f"{float(np.mean(res['iters'])):.0f}, max {int(np.max(res['iters']))}")
-    value = res["converged"] * args.steps / total
+        total_cold += r["seconds"]
+        res_cold = b.download()
+        log(f"bench: cold step {i}: {r['seconds']:.3f} s, {r['sweeps']} sweeps, "
+            f"{res_cold['converged']}/{args.n_inst} converged, mean iters "
+            f"{float(np.mean(res_cold['iters'])):.1f}, max {int(np.max(res_cold['iters']))}")
+    value_cold = res_cold["converged"] * args.steps / total_cold
+
+    # ---- warm start: the MLP trained on fields the batched solver produced
+    warm = None
+    if not args.no_warm:
+        warm = warm_start_leg(S, WL, b, cfg, w0, args)
+    res = warm["res"] if warm else res_cold
+    total = warm["total"] if warm else total_cold
+    value = warm["value"] if warm else value_cold
     gb_eff = res["bytes"] / (total / args.steps) / 1e9
 
     out = {
@@ -212,14 +304,21 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"config3: {args.n_inst} DN instances, m log-uniform [32,4096], "
-                        "N in {512,1024,2048,4096}, L in [2,8], cold start, tol 1e-6",
+                        "N in {512,1024,2048,4096}, L in [2,8], tol 1e-6",
             "n_inst": args.n_inst,
-            "start": "cold",
+            "start": "warm (MLP) with cold fallback" if warm else "cold",
             "solver": SOLVER_TEXT,
             "converged": int(res["converged"]),
-            "mean_iters": round(float(np.mean(res["iters"])), 1),
+            "mean_iters": round(float(np.mean(res["iters"])), 2),
             "max_iters": int(np.max(res["iters"])),
             "solve_gbs_algorithmic": round(gb_eff, 1),
+            "cold_start": {
+                "value": round(value_cold, 3), "unit": "solves/s",
+                "ms_per_step": round(1e3 * total_cold / args.steps, 3),
+                "converged": int(res_cold["converged"]),
+                "mean_iters": round(float(np.mean(res_cold["iters"])), 2),
+                "max_iters": int(np.max(res_cold["iters"])),
+            },
         },
         "roofline": {
             "bound": "hbm",
@@ -231,9 +330,12 @@ def main():
             "kernel": f"{kern}: bucket of {n_items} instances, {ms:.3f} ms per launch",
         },
     }
+    if warm:
+        out["config"]["warm_start"] = warm["info"]
+        out["config"]["warm_over_cold"] = round(value / value_cold, 3)
     if not args.no_cpu:
         threads = min(16, len(os.sched_getaffinity(0)))  # one GPU's share of the host
-        cb = cpu_baseline(cfg, res["iters"], args.max_iter, threads, args.cpu_budget)
+        cb = cpu_baseline(cfg, res_cold["iters"], args.max_iter, threads, args.cpu_budget)
         out["cpu_baseline"] = {
             "value": round(cb["value"], 4), "unit": "solves/s", "cores": threads,
             "kind": "port",

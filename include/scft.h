@@ -243,6 +243,14 @@ int scft_mlp_train(scft_mlp *m, int n_samples, const float *X, const float *Y,
  * reference's dense gaussj. */
 int scft_resample_spline(int n_src, const double *x_src, const double *y_src,
                          int n_dst, const double *x_dst, double *y_dst);
+/* The same for n_tables independent tables in one call: table k has
+ * n_src[k] points at x_src + src_off[k] (values y_src + src_off[k]) and is
+ * evaluated at the n_dst[k] points x_dst + dst_off[k] into y_dst + dst_off[k]. */
+int scft_resample_spline_batch(int n_tables, const int *n_src,
+                               const long long *src_off, const double *x_src,
+                               const double *y_src, const int *n_dst,
+                               const long long *dst_off, const double *x_dst,
+                               double *y_dst);
 
 #ifdef __cplusplus
 }

@@ -122,6 +122,7 @@ def _load():
     L.scft_mlp_train.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_float,
                                  C.c_ulonglong, C.c_void_p]
     L.scft_resample_spline.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    L.scft_resample_spline_batch.argtypes = [C.c_int, _ip, _lp, _dp, _dp, _ip, _lp, _dp, _dp]
     return L
 
 
@@ -230,6 +231,10 @@ class Batch:
         return [flat[o:o + n] for o, n in zip(self.uoff, self.n)]
 
     def join(self, parts):
+        """Concatenated unknowns of all instances (a flat array passes through)."""
+        if isinstance(parts, np.ndarray) and parts.ndim == 1:
+            assert parts.size == self.total
+            return _f64(parts)
         return _f64(np.concatenate(parts))
 
     def eval(self, w_parts):
@@ -327,6 +332,36 @@ def resample_spline(x_src, y_src, x_dst):
     out = np.empty(len(xd))
     check(lib.scft_resample_spline(len(xs), xs, ys, len(xd), xd, out), "scft_resample_spline")
     return out
+
+
+def _tables(arrs):
+    """(concatenated values, lengths, offsets) of a list of 1-D arrays or of
+    the rows of a 2-D array."""
+    if isinstance(arrs, np.ndarray) and arrs.ndim == 2:
+        n = np.full(arrs.shape[0], arrs.shape[1], dtype=np.int32)
+        return _f64(arrs.reshape(-1)), n, np.arange(arrs.shape[0], dtype=np.int64) * arrs.shape[1]
+    n = np.array([len(v) for v in arrs], dtype=np.int32)
+    off = np.zeros(len(n), dtype=np.int64)
+    off[1:] = np.cumsum(n[:-1])
+    return _f64(np.concatenate(arrs)), n, off
+
+
+def resample_spline_batch(xs_src, ys_src, xs_dst, flat=False):
+    """resample_spline for many tables in one library call.  Each argument is
+    a list of 1-D arrays or a 2-D array (one table per row).  Returns the
+    concatenated result (n_total,) when the destinations form a 2-D array or
+    flat=True (with the offsets for flat=True), else a list of arrays."""
+    xs, ns, so = _tables(xs_src)
+    ys, _, _ = _tables(ys_src)
+    xd, nd, do = _tables(xs_dst)
+    out = np.empty(int(nd.sum()))
+    check(lib.scft_resample_spline_batch(len(ns), ns, so, xs, ys, nd, do, xd, out),
+          "scft_resample_spline_batch")
+    if flat:
+        return out, do
+    if isinstance(xs_dst, np.ndarray) and xs_dst.ndim == 2:
+        return out
+    return [out[o:o + n] for o, n in zip(do, nd)]
 
 
 # ---------------------------------------------------------------- MLP

@@ -11,6 +11,7 @@
 #include <fstream>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scft.h"
@@ -995,5 +996,46 @@ extern "C" int scft_resample_spline(int n_src, const double* x_src, const double
   std::vector<double> M;
   if (!scft::spline_not_a_knot(n_src, x_src, y_src, M)) return SCFT_E_ARG;
   for (int i = 0; i < n_dst; i++) y_dst[i] = scft::spline_eval(n_src, x_src, y_src, M.data(), x_dst[i]);
+  return SCFT_OK;
+}
+
+extern "C" int scft_resample_spline_batch(int n_tables, const int* n_src,
+                                          const long long* src_off, const double* x_src,
+                                          const double* y_src, const int* n_dst,
+                                          const long long* dst_off, const double* x_dst,
+                                          double* y_dst) {
+  if (n_tables < 0 || (n_tables > 0 && (!n_src || !src_off || !x_src || !y_src || !n_dst ||
+                                        !dst_off || !x_dst || !y_dst)))
+    return SCFT_E_ARG;
+  // tables are independent: contiguous ranges of roughly equal work on up
+  // to 16 host threads
+  double total = 0.0;
+  for (int k = 0; k < n_tables; k++) total += n_src[k] + n_dst[k];
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const int nt = (total < 2e5) ? 1 : static_cast<int>(std::min<unsigned>(hw, n_tables));
+  std::vector<int> bounds(1, 0);
+  double acc = 0.0;
+  for (int k = 0; k < n_tables && static_cast<int>(bounds.size()) < nt; k++) {
+    acc += n_src[k] + n_dst[k];
+    if (acc >= total * bounds.size() / nt) bounds.push_back(k + 1);
+  }
+  bounds.push_back(n_tables);
+  std::vector<int> rcs(bounds.size(), SCFT_OK);
+  auto work = [&](size_t part) {
+    for (int k = bounds[part]; k < bounds[part + 1]; k++) {
+      const int rc = scft_resample_spline(n_src[k], x_src + src_off[k], y_src + src_off[k],
+                                          n_dst[k], x_dst + dst_off[k], y_dst + dst_off[k]);
+      if (rc != SCFT_OK) {
+        rcs[part] = rc;
+        return;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (size_t p = 1; p + 1 < bounds.size(); p++) pool.emplace_back(work, p);
+  work(0);
+  for (auto& t : pool) t.join();
+  for (int rc : rcs)
+    if (rc != SCFT_OK) return rc;
   return SCFT_OK;
 }
