@@ -205,6 +205,30 @@ int scft_batch_autotune(scft_batch *b, int reps);
 int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
                       int *K, int *W, int max_buckets);
 
+/* ---- Broyden with a batched Jacobian ------------------------------------
+ * Replaces broydn(x, n, &check, simple_FEM_1D_transient) (1D_FEM.c:356,
+ * broydn.c:44-292 with DEALII_SCFT/src/{fdjac,lnsrch,qrdcmp,qrupdt,rotate,
+ * rsolv}.c, same constants) for one instance: F(w) = phi[w] - phi_t, the
+ * n-column forward-difference Jacobian of fdjac.c:9-41 evaluated as ONE
+ * batched sweep of n perturbed fields, the other evaluations on the device
+ * one at a time, the n x n QR / rank-1 updates on the host (O(n^3) per
+ * Jacobian: meant for n up to ~1000).  Instance as in scft_batch_desc (M
+ * nodes at x, N steps, phi_t on the n unknowns, bc, cmat); w on the unknowns
+ * (0-based): in = start, out = result; tolf = the reference's `err`.
+ * Returns SCFT_OK when broydn's check would be 0, SCFT_E_NOCONV where it
+ * would be 1 (or where the reference would exit), SCFT_E_NAN / SCFT_E_PIVOT
+ * for a failed evaluation / singular Jacobian. */
+typedef struct scft_broyden_stats {
+  int iterations;     /* Broyden iterations */
+  long evaluations;   /* residual evaluations, Jacobian columns included */
+  long sweeps;        /* device launches: one per single evaluation, one per Jacobian */
+  double final_err;   /* ||F||_inf at return */
+  int check;          /* broydn's check flag */
+} scft_broyden_stats;
+int scft_broyden_solve(int M, int N, const double *x, const double *phi_t,
+                       int bc, int cmat, double tolf, double *w,
+                       scft_broyden_stats *stats);
+
 /* ---- warm-start MLP (no reference counterpart) -------------------------
  * Dense FP32 network trained and evaluated on the device by hand-written
  * kernels (matrix-core GEMMs with fused bias / tanh / tanh' epilogues, MSE,

@@ -64,6 +64,11 @@ class AndersonOpts(C.Structure):
                 ("backtrack", C.c_double), ("max_halvings", C.c_int)]
 
 
+class BroydenStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("evaluations", C.c_long), ("sweeps", C.c_long),
+                ("final_err", C.c_double), ("check", C.c_int)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("converged", C.c_int), ("seconds", C.c_double),
                 ("eval_seconds", C.c_double), ("node_steps", C.c_double), ("bytes", C.c_double)]
@@ -122,6 +127,8 @@ def _load():
     L.scft_mlp_train.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_float,
                                  C.c_ulonglong, C.c_void_p]
     L.scft_resample_spline.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    L.scft_broyden_solve.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double,
+                                     _dp, C.POINTER(BroydenStats)]
     L.scft_resample_spline_batch.argtypes = [C.c_int, _ip, _lp, _dp, _dp, _ip, _lp, _dp, _dp]
     return L
 
@@ -436,3 +443,15 @@ class Mlp:
         check(lib.scft_mlp_train(self.h, X.shape[0], X, Y, epochs, batch, lr, seed,
                                  hist.ctypes.data), "scft_mlp_train")
         return hist
+
+
+# ---------------------------------------------------------------- Broyden
+def broyden_solve(x, phi_t_unknowns, N, w_unknowns, tolf=1e-8, bc=BC_DN, cmat=C_ROWSCALED):
+    """Broyden (broydn.c semantics) on one instance with the batched
+    finite-difference Jacobian; returns the field on the unknowns and the
+    statistics (rc 0 = converged)."""
+    x, pt, w = _f64(x), _f64(phi_t_unknowns), _f64(w_unknowns).copy()
+    st = BroydenStats()
+    rc = lib.scft_broyden_solve(len(x), int(N), x, pt, bc, cmat, tolf, w, C.byref(st))
+    return dict(rc=rc, w=w, iterations=st.iterations, evaluations=st.evaluations,
+                sweeps=st.sweeps, final_err=st.final_err, check=st.check)

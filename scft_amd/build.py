@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libscft_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
-SOURCES = ["host_math.cpp", "march.hip", "anderson.hip", "mlp.hip", "capi.hip"]
+SOURCES = ["host_math.cpp", "march.hip", "anderson.hip", "mlp.hip", "capi.hip", "broyden.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           f"--offload-arch={ARCH}",
