@@ -229,6 +229,48 @@ int scft_broyden_solve(int M, int N, const double *x, const double *phi_t,
                        int bc, int cmat, double tolf, double *w,
                        scft_broyden_stats *stats);
 
+/* ---- reference file formats ---------------------------------------------
+ * .res files (Exp_m32_n2048_IE.res: the reader of 1D_FEM.c:321-342 skips
+ * 10 lines and takes column 3 as the interior field): header values and
+ * every table row (x/l, phi, eta, phie, phij), the wall rows included, so
+ * rows 1..M-2 are 1D_FEM.c's yita_middle.  *rows receives the row count
+ * (arrays nullable, at most max_rows filled). */
+typedef struct scft_res_header {
+  int m, n;        /* elements, contour steps ("m = 32   n = 2048") */
+  int iters;       /* "N=1000" */
+  double l, cp, mphi, t, Z, f, fmax;
+} scft_res_header;
+int scft_read_res(const char *path, int max_rows, double *x_over_l, double *phi,
+                  double *eta, double *phie, double *phij, int *rows,
+                  scft_res_header *hdr);
+/* solution_yita_1D_N=*.txt (DEALII_SCFT/src/scft.cc:318-334): "N= M, ERROR=
+ * e", "mean_field_free_energy, F", then "i,x_i,w_i" per node. */
+int scft_write_solution(const char *path, int M, const double *x,
+                        const double *w, double err, double free_energy);
+/* Reader of that format (scft_util.cc:13-41); files without the free-energy
+ * line (older MATLAB output) are accepted and *free_energy is NaN. */
+int scft_read_solution(const char *path, int cap, int *M, double *x, double *w,
+                       double *err, double *free_energy);
+/* Mean-field free energy (DEALII_SCFT/src/scft.cc:404-450): the field linear
+ * between its M nodes, times the position-based target (mirrored when
+ * `mirror`), Romberg-integrated on 2^log2_points + 1 uniform points of
+ * [0, L] (the reference uses 2^18 + 1); F = (I / f0bar / L + ln f0bar) /
+ * -1000 with f0bar = 0.892581217773656 (scft.cc:414). */
+int scft_free_energy(int M, const double *x, const double *w_nodes, double L,
+                     double tau, int mirror, int log2_points, double *F);
+
+/* ---- mesh continuation ---------------------------------------------------
+ * One refinement by the rule of Matlab_files/refine_mesh.m:1-36: cell
+ * errors |dw/dx| (wall cells +inf), cells with error >= factor * median
+ * (the reference: 10) bisected; the field moved to the new unknowns by the
+ * not-a-knot spline through the old ones (refine_mesh.m:33, spline_chen).
+ * *M_new receives the new node count; x_new / w_new (capacity cap) the new
+ * nodes and field (wall values kept); with x_new or w_new NULL only the count
+ * is returned.  bc: the far end is a wall for DD. */
+int scft_refine_mesh(int M, const double *x, const double *w_nodes,
+                     double factor, int bc, int cap, int *M_new, double *x_new,
+                     double *w_new);
+
 /* ---- warm-start MLP (no reference counterpart) -------------------------
  * Dense FP32 network trained and evaluated on the device by hand-written
  * kernels (matrix-core GEMMs with fused bias / tanh / tanh' epilogues, MSE,

@@ -69,6 +69,12 @@ class BroydenStats(C.Structure):
                 ("final_err", C.c_double), ("check", C.c_int)]
 
 
+class ResHeader(C.Structure):
+    _fields_ = [("m", C.c_int), ("n", C.c_int), ("iters", C.c_int), ("l", C.c_double),
+                ("cp", C.c_double), ("mphi", C.c_double), ("t", C.c_double), ("Z", C.c_double),
+                ("f", C.c_double), ("fmax", C.c_double)]
+
+
 class SolveStats(C.Structure):
     _fields_ = [("sweeps", C.c_int), ("converged", C.c_int), ("seconds", C.c_double),
                 ("eval_seconds", C.c_double), ("node_steps", C.c_double), ("bytes", C.c_double)]
@@ -127,6 +133,15 @@ def _load():
     L.scft_mlp_train.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.c_int, C.c_int, C.c_float,
                                  C.c_ulonglong, C.c_void_p]
     L.scft_resample_spline.argtypes = [C.c_int, _dp, _dp, C.c_int, _dp, _dp]
+    L.scft_read_res.argtypes = [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _dp,
+                                C.POINTER(C.c_int), C.POINTER(ResHeader)]
+    L.scft_write_solution.argtypes = [C.c_char_p, C.c_int, _dp, _dp, C.c_double, C.c_double]
+    L.scft_read_solution.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), _dp, _dp,
+                                     C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.scft_free_energy.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                   C.POINTER(C.c_double)]
+    L.scft_refine_mesh.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_int,
+                                   C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
     L.scft_broyden_solve.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                      _dp, C.POINTER(BroydenStats)]
     L.scft_resample_spline_batch.argtypes = [C.c_int, _ip, _lp, _dp, _dp, _ip, _lp, _dp, _dp]
@@ -455,3 +470,51 @@ def broyden_solve(x, phi_t_unknowns, N, w_unknowns, tolf=1e-8, bc=BC_DN, cmat=C_
     rc = lib.scft_broyden_solve(len(x), int(N), x, pt, bc, cmat, tolf, w, C.byref(st))
     return dict(rc=rc, w=w, iterations=st.iterations, evaluations=st.evaluations,
                 sweeps=st.sweeps, final_err=st.final_err, check=st.check)
+
+
+# ---------------------------------------------------------------- file formats
+def read_res(path, max_rows=1 << 16):
+    """Columns and header of a .res file (1D_FEM.c:321-342 format)."""
+    cols = [np.empty(max_rows) for _ in range(5)]
+    rows, h = C.c_int(), ResHeader()
+    check(lib.scft_read_res(str(path).encode(), max_rows, *cols, C.byref(rows), C.byref(h)),
+          "scft_read_res")
+    r = rows.value
+    out = {k: c[:r].copy() for k, c in zip(("x_over_l", "phi", "eta", "phie", "phij"), cols)}
+    out["header"] = {k: getattr(h, k) for k, _ in ResHeader._fields_}
+    return out
+
+
+def write_solution(path, x, w, err, free_energy):
+    x, w = _f64(x), _f64(w)
+    check(lib.scft_write_solution(str(path).encode(), len(x), x, w, err, free_energy),
+          "scft_write_solution")
+
+
+def read_solution(path, cap=1 << 20):
+    x, w = np.empty(cap), np.empty(cap)
+    M, e, fe = C.c_int(), C.c_double(), C.c_double()
+    check(lib.scft_read_solution(str(path).encode(), cap, C.byref(M), x, w, C.byref(e),
+                                 C.byref(fe)), "scft_read_solution")
+    return dict(x=x[:M.value].copy(), w=w[:M.value].copy(), err=e.value, free_energy=fe.value)
+
+
+def free_energy(x, w_nodes, L=None, tau=TAU, mirror=True, log2_points=18):
+    """Mean-field free energy of DEALII_SCFT/src/scft.cc:404-450."""
+    x, w = _f64(x), _f64(w_nodes)
+    F = C.c_double()
+    check(lib.scft_free_energy(len(x), x, w, float(L if L is not None else x[-1]), tau,
+                               1 if mirror else 0, log2_points, C.byref(F)), "scft_free_energy")
+    return F.value
+
+
+def refine_mesh(x, w_nodes, factor=10.0, bc=BC_DN):
+    """One gradient-rule refinement with spline transfer of the field."""
+    x, w = _f64(x), _f64(w_nodes)
+    Mn = C.c_int()
+    check(lib.scft_refine_mesh(len(x), x, w, factor, bc, 0, C.byref(Mn), None, None),
+          "scft_refine_mesh")
+    xn, wn = np.empty(Mn.value), np.empty(Mn.value)
+    check(lib.scft_refine_mesh(len(x), x, w, factor, bc, Mn.value, C.byref(Mn),
+                               xn.ctypes.data, wn.ctypes.data), "scft_refine_mesh")
+    return xn, wn

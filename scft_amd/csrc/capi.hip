@@ -1039,3 +1039,139 @@ extern "C" int scft_resample_spline_batch(int n_tables, const int* n_src,
     if (rc != SCFT_OK) return rc;
   return SCFT_OK;
 }
+
+// ------------------------------------------------- reference file formats
+extern "C" int scft_read_res(const char* path, int max_rows, double* x_over_l, double* phi,
+                             double* eta, double* phie, double* phij, int* rows,
+                             scft_res_header* hdr) {
+  if (!path || max_rows < 0 || !rows) return SCFT_E_ARG;
+  FILE* f = fopen(path, "r");
+  if (!f) return SCFT_E_ARG;
+  char buf[512];
+  scft_res_header h{};
+  int r = 0, line = 0;
+  bool table = false;
+  while (fgets(buf, sizeof buf, f)) {
+    line++;
+    if (!table) {
+      int a, b, nn;
+      double v1, v2, v3, v4, v5;
+      if (sscanf(buf, " m = %d n = %d", &a, &b) == 2) {
+        h.m = a;
+        h.n = b;
+      } else if (sscanf(buf, " N=%d l=%lf cp=%lf mphi=%lf t=%lf", &nn, &v1, &v2, &v3, &v4) == 5) {
+        h.iters = nn;
+        h.l = v1;
+        h.cp = v2;
+        h.mphi = v3;
+        h.t = v4;
+      } else if (sscanf(buf, " Z=%lf f=%lf |f|max=%lf", &v1, &v2, &v3) == 3) {
+        h.Z = v1;
+        h.f = v2;
+        h.fmax = v3;
+      } else if (buf[0] == '-') {
+        table = true;  // the dashes under the column names
+      }
+      (void)v5;
+      continue;
+    }
+    double c[5];
+    if (sscanf(buf, "%lf %lf %lf %lf %lf", &c[0], &c[1], &c[2], &c[3], &c[4]) != 5) continue;
+    if (r < max_rows) {
+      if (x_over_l) x_over_l[r] = c[0];
+      if (phi) phi[r] = c[1];
+      if (eta) eta[r] = c[2];
+      if (phie) phie[r] = c[3];
+      if (phij) phij[r] = c[4];
+    }
+    r++;
+  }
+  fclose(f);
+  *rows = r;
+  if (hdr) *hdr = h;
+  return (table && r > 0) ? SCFT_OK : SCFT_E_ARG;
+}
+
+extern "C" int scft_write_solution(const char* path, int M, const double* x, const double* w,
+                                   double err, double free_energy) {
+  if (!path || M < 1 || !x || !w) return SCFT_E_ARG;
+  FILE* f = fopen(path, "w");
+  if (!f) return SCFT_E_ARG;
+  fprintf(f, "N= %d, ", M);
+  fprintf(f, "ERROR= %e \n", err);
+  fprintf(f, "mean_field_free_energy, %2.15f \n", free_energy);
+  for (int i = 0; i < M; i++) fprintf(f, "%d,%2.15f,%2.15f\n", i, x[i], w[i]);
+  const bool ok = fflush(f) == 0;
+  fclose(f);
+  return ok ? SCFT_OK : SCFT_E_ARG;
+}
+
+extern "C" int scft_read_solution(const char* path, int cap, int* M, double* x, double* w,
+                                  double* err, double* free_energy) {
+  if (!path || cap < 0 || !M) return SCFT_E_ARG;
+  FILE* f = fopen(path, "r");
+  if (!f) return SCFT_E_ARG;
+  char buf[512];
+  int n = 0;
+  double e = NAN, fe = NAN;
+  if (!fgets(buf, sizeof buf, f) || sscanf(buf, "N= %d", &n) != 1) {
+    fclose(f);
+    return SCFT_E_ARG;
+  }
+  const char* pe = strstr(buf, "ERROR=");
+  if (pe) sscanf(pe, "ERROR= %lf", &e);
+  int count = 0;
+  while (fgets(buf, sizeof buf, f)) {
+    if (strncmp(buf, "mean_field_free_energy", 22) == 0) {
+      sscanf(buf, "mean_field_free_energy, %lf", &fe);
+      continue;
+    }
+    int i;
+    double xv, wv;
+    if (sscanf(buf, "%d ,%lf, %lf", &i, &xv, &wv) != 3) continue;
+    if (i < 0 || i >= n) continue;
+    if (i < cap) {
+      if (x) x[i] = xv;
+      if (w) w[i] = wv;
+    }
+    count++;
+  }
+  fclose(f);
+  *M = n;
+  if (err) *err = e;
+  if (free_energy) *free_energy = fe;
+  return (count == n) ? SCFT_OK : SCFT_E_ARG;
+}
+
+extern "C" int scft_free_energy(int M, const double* x, const double* w_nodes, double L,
+                                double tau, int mirror, int log2_points, double* F) {
+  if (M < 2 || !x || !w_nodes || !F || !(L > 0.0) || log2_points < 4 || log2_points > 24)
+    return SCFT_E_ARG;
+  return scft::free_energy(M, x, w_nodes, L, tau, mirror != 0, log2_points, 0.892581217773656, F)
+             ? SCFT_OK
+             : SCFT_E_ARG;
+}
+
+// ------------------------------------------------------- mesh continuation
+extern "C" int scft_refine_mesh(int M, const double* x, const double* w_nodes, double factor,
+                                int bc, int cap, int* M_new, double* x_new, double* w_new) {
+  if (M < 5 || !x || !w_nodes || !M_new || !(factor > 0.0)) return SCFT_E_ARG;
+  std::vector<double> wpad(w_nodes, w_nodes + M);
+  const std::vector<double> xn = scft::refine_by_gradient(M, x, wpad.data(), factor, bc == SCFT_BC_DD);
+  const int Mn = static_cast<int>(xn.size());
+  *M_new = Mn;
+  if (!x_new || !w_new) return SCFT_OK;  // size query
+  if (Mn > cap) return SCFT_E_ARG;
+  std::copy(xn.begin(), xn.end(), x_new);
+  // the field on the new unknowns: not-a-knot spline through the old
+  // unknowns (refine_mesh.m:33, spline_chen); the walls keep their values
+  const int u_end = (bc == SCFT_BC_DD) ? M - 1 : M;
+  const int un_end = (bc == SCFT_BC_DD) ? Mn - 1 : Mn;
+  std::vector<double> Ms;
+  if (!scft::spline_not_a_knot(u_end - 1, x + 1, w_nodes + 1, Ms)) return SCFT_E_ARG;
+  for (int i = 1; i < un_end; i++)
+    w_new[i] = scft::spline_eval(u_end - 1, x + 1, w_nodes + 1, Ms.data(), xn[i]);
+  w_new[0] = w_nodes[0];
+  if (bc == SCFT_BC_DD) w_new[Mn - 1] = w_nodes[M - 1];
+  return SCFT_OK;
+}

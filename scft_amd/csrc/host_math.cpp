@@ -2,6 +2,7 @@
 // weights, the EXP target profile and uniform meshes.
 #include "host_math.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -151,6 +152,65 @@ double spline_eval(int n, const double* x, const double* y, const double* M, dou
   const double a = (x[khi] - xp) / h, b = (xp - x[klo]) / h;
   return a * y[klo] + b * y[khi] +
          ((a * a * a - a) * M[klo] + (b * b * b - b) * M[khi]) * (h * h) / 6.0;
+}
+
+}  // namespace scft
+
+namespace scft {
+
+// Cells flagged by the gradient rule of Matlab_files/refine_mesh.m:1-36:
+// e_c = |w_{c+1} - w_c| / (x_{c+1} - x_c) over the cells of the node field
+// (cells touching a Dirichlet wall count as +inf, as the reference pads the
+// interior solution with Inf; the far end is a wall only for DD), refined
+// when e_c >= factor * median(e).  Returns the new nodes (flagged cells
+// bisected).
+std::vector<double> refine_by_gradient(int M, const double* x, const double* w_nodes,
+                                       double factor, bool wall_right) {
+  std::vector<double> e(M - 1);
+  for (int c = 0; c < M - 1; c++) {
+    const bool wall = (c == 0 || (wall_right && c == M - 2));
+    e[c] = wall ? INFINITY : std::fabs((w_nodes[c + 1] - w_nodes[c]) / (x[c + 1] - x[c]));
+  }
+  std::vector<double> s(e);
+  std::sort(s.begin(), s.end());
+  // MATLAB median: mean of the two middle values for an even count
+  const size_t k = s.size();
+  const double med = (k % 2) ? s[k / 2] : 0.5 * (s[k / 2 - 1] + s[k / 2]);
+  const double threshold = factor * med;
+  std::vector<double> xn;
+  xn.reserve(2 * M);
+  for (int c = 0; c < M - 1; c++) {
+    xn.push_back(x[c]);
+    if (e[c] >= threshold) xn.push_back(0.5 * (x[c] + x[c + 1]));
+  }
+  xn.push_back(x[M - 1]);
+  return xn;
+}
+
+// Mean-field free energy of DEALII_SCFT/src/scft.cc:404-450
+// (set_mean_field_free_energy_romint): the field, linear between its nodes
+// (the finite-element field sampled at y = 0), times the position-based
+// target (mirrored at the far wall when `mirror`), on 2^k + 1 uniform points
+// of [0, L]; Romberg (romint) over them; F = (I / f0bar / L + ln f0bar) / -1000.
+bool free_energy(int M, const double* x, const double* w_nodes, double L, double tau, bool mirror,
+                 int k, double f0bar, double* F) {
+  const int m = 1 << k;
+  std::vector<double> c;
+  if (!romberg_weights(m, L / m, c)) return false;
+  std::vector<double> xp(m + 1), ft(m + 1);
+  for (int i = 0; i <= m; i++) xp[i] = L * i / m;
+  target_position(m + 1, xp.data(), tau, mirror, ft.data());
+  double I = 0.0;
+  int seg = 0;
+  for (int i = 0; i <= m; i++) {
+    const double xi = xp[i];
+    while (seg + 2 < M && x[seg + 1] < xi) seg++;
+    const double t = (xi - x[seg]) / (x[seg + 1] - x[seg]);
+    const double wv = w_nodes[seg] + t * (w_nodes[seg + 1] - w_nodes[seg]);
+    I += c[i] * wv * ft[i];
+  }
+  *F = (I / f0bar / L + std::log(f0bar)) / (-1000.0);
+  return true;
 }
 
 }  // namespace scft

@@ -110,6 +110,16 @@ def main():
     with open(path, "w") as f:
         json.dump(fx, f)
     print(path, os.path.getsize(path), "bytes")
+    # data files the readers are tested on, byte for byte (no spaces in names)
+    import shutil
+    files = os.path.join(HERE, "files")
+    os.makedirs(files, exist_ok=True)
+    for src, dst in (("Exp_m32_n2048_IE.res", "Exp_m32_n2048_IE.res"),
+                     ("DEALII_SCFT/inputFiles/N=33_for_read.txt", "N=33_for_read.txt"),
+                     ("Matlab_files/inputFiles/solution_yita_1D_N= 141.txt",
+                      "solution_yita_1D_N=141.txt")):
+        shutil.copyfile(os.path.join(REF, src), os.path.join(files, dst))
+        print(os.path.join(files, dst))
 
 
 if __name__ == "__main__":
