@@ -37,6 +37,14 @@ extern "C" {
 #define SCFT_BC_DN 1
 #define SCFT_C_ROWSCALED 0  /* C_ij = w_i A_ij (1D_FEM.c:104-105)            */
 #define SCFT_C_CONSISTENT 1 /* C_ij = int w_h phi_i phi_j (scft.cc:641-651)   */
+/* contour stepper of the propagator march */
+#define SCFT_STEP_IE 0   /* implicit Euler (1D_FEM.c:208-228, IE march of the MATLAB files) */
+#define SCFT_STEP_IRK4 1 /* two-stage Gauss-Legendre (DEALII_SCFT/src/scft.cc:671-695,
+                            drivescft.cc:130-146, Matlab_files/IRK4.m:96-120): fourth
+                            order in s, A- but not L-stable (the step factor of a mode
+                            tends to 1 as dt*lambda grows), so meant for
+                            dt * 12/h_min^2 up to ~25 as in the reference (33 nodes,
+                            N = 2048); at most 2048 unknowns per instance */
 
 /* per-instance status word bits (scft_batch_* outputs) */
 #define SCFT_ST_CONVERGED 1u
@@ -59,6 +67,7 @@ typedef struct scft_config {
   int cmat;          /* SCFT_C_ROWSCALED (default) / SCFT_C_CONSISTENT      */
   double beta;       /* mixing used by scft_fixed_point (default 0.1)       */
   int verbose;       /* print each evaluation like 1D_FEM.c:280-283 (default 0) */
+  int stepper;       /* SCFT_STEP_IE (default) / SCFT_STEP_IRK4 (the deal.II path) */
 } scft_config;
 
 void scft_default_config(scft_config *cfg);
@@ -115,6 +124,8 @@ typedef struct scft_batch_desc {
   int bc;              /* SCFT_BC_DD / SCFT_BC_DN (all instances)           */
   int cmat;            /* SCFT_C_ROWSCALED / SCFT_C_CONSISTENT              */
   const char *tune_file; /* autotuner table to read/write, or NULL          */
+  int stepper;         /* SCFT_STEP_IE (0, also a zero-initialised desc) /
+                          SCFT_STEP_IRK4; a tune file holds one stepper's table */
 } scft_batch_desc;
 
 int scft_batch_create(scft_batch **out, const scft_batch_desc *desc);
@@ -126,6 +137,13 @@ int scft_batch_layout(const scft_batch *b, long long *uoff, long long *total);
  * phi (nullable), r = phi - phi_t, rnorm[k] = ||r_k||_inf, status[k]. */
 int scft_batch_eval(scft_batch *b, const double *w, double *phi, double *r,
                     double *rnorm, unsigned *status);
+
+/* Field values at the two walls, used by the consistent w-mass
+ * (SCFT_C_CONSISTENT) of the wall elements: the deal.II path extrapolates
+ * them from the interior field by a natural spline (DEALII_SCFT/src/scft.cc:
+ * 454-480); the default is 0, the convention of 1D_FEM.c:65-66.  One value
+ * per instance in each array (wallL is not read under SCFT_BC_DN). */
+int scft_batch_set_walls(scft_batch *b, const double *wall0, const double *wallL);
 
 /* Diagnostic for parity tests: the q-slabs 1..N/2-1 kept in device scratch
  * by the last evaluation of a single-instance batch, in node order

@@ -24,6 +24,7 @@ REFERENCE_ROOT = "/root/reference"
 
 BC_DD, BC_DN = 0, 1
 C_ROWSCALED, C_CONSISTENT = 0, 1
+STEP_IE, STEP_IRK4 = 0, 1
 
 # Reference constants: tau and L at full precision (DEALII_SCFT/drivescft.cc:
 # 269-271, Exp_m32_n2048_IE.res:3) and as typed in 1D_FEM.c:303 / drive_SCFT.m.
@@ -94,6 +95,9 @@ def lib():
         L.orc_thomas.argtypes = [C.c_int, _dp, _dp, _dp]
         L.orc_march.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
         L.orc_march_ld.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_march_irk4.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, _dp]
+        L.orc_eval_stepper.argtypes = [C.POINTER(OrcProblem), C.c_int, _dp, C.c_void_p, _dp,
+                                       _dp, C.POINTER(C.c_double)]
         L.orc_romint.restype = C.c_double
         L.orc_romint.argtypes = [_dp, C.c_int, C.c_double]
         L.orc_romberg_weights.argtypes = [C.c_int, C.c_double, _dp]
@@ -200,6 +204,18 @@ def march(x, w, N, bc=BC_DD, cmat=C_ROWSCALED):
     return q.reshape(N + 1, M)
 
 
+def march_irk4(x, w, N, bc=BC_DD, cmat=C_ROWSCALED):
+    """Two-stage Gauss-Legendre march (the stage system as a 2x2-block
+    tridiagonal solve); slabs (N+1, M)."""
+    x, w = _f64(x), _f64(w)
+    M = len(x)
+    q = np.empty((N + 1) * M)
+    rc = lib().orc_march_irk4(M, N, x, w, bc, cmat, q)
+    if rc != 0:
+        raise RuntimeError(f"orc_march_irk4 failed rc={rc}")
+    return q.reshape(N + 1, M)
+
+
 def march_ld(x, w, N, bc=BC_DD, cmat=C_ROWSCALED):
     """The march in x87 extended precision (the higher-precision truth)."""
     x, w = _f64(x), _f64(w)
@@ -248,15 +264,17 @@ def make_problem(x, phi_t, N, bc=BC_DD, cmat=C_ROWSCALED):
     return p
 
 
-def evaluate(x, w, phi_t, N, bc=BC_DD, cmat=C_ROWSCALED, want_q=False):
-    """Mode-R evaluation; returns dict(phi, r, rnorm[, q])."""
+def evaluate(x, w, phi_t, N, bc=BC_DD, cmat=C_ROWSCALED, want_q=False, stepper=STEP_IE):
+    """Mode-R evaluation (implicit Euler, or IRK4 with stepper=STEP_IRK4);
+    returns dict(phi, r, rnorm[, q])."""
     p = make_problem(x, phi_t, N, bc, cmat)
     M = len(x)
     w = _f64(w)
     phi, r = np.empty(M), np.empty(M)
     nrm = C.c_double()
     q = np.empty((N + 1) * M) if want_q else None
-    rc = lib().orc_eval(C.byref(p), w, q.ctypes.data if want_q else None, phi, r, C.byref(nrm))
+    rc = lib().orc_eval_stepper(C.byref(p), stepper, w, q.ctypes.data if want_q else None, phi, r,
+                                C.byref(nrm))
     out = dict(phi=phi, r=r, rnorm=nrm.value, rc=rc)
     if want_q:
         out["q"] = q.reshape(N + 1, M)

@@ -413,14 +413,16 @@ int orc_first_unknown(void) { return 1; }
  * nodes (1D_FEM.c:274-277 sign; zero on Dirichlet nodes), and the infinity
  * norm over the unknowns (adm.c:114-122).  q_out (nullable) receives the
  * (N+1)*M slabs. */
-int orc_eval(const orc_problem *p, const double *w, double *q_out,
-             double *phi, double *r, double *rnorm) {
+static int eval_stepper(const orc_problem *p, int stepper, const double *w,
+                        double *q_out, double *phi, double *r, double *rnorm) {
   int M = p->M, N = p->N;
   double *q = q_out ? q_out : (double *)malloc(sizeof(double) * (size_t)(N + 1) * M);
   double *c = (double *)malloc(sizeof(double) * (N + 1));
   if (!q || !c) return ORC_E_ALLOC;
   int rc = orc_romberg_weights(N, 1.0 / N, c);
-  if (rc == ORC_OK) rc = orc_march(M, N, p->x, w, p->bc, p->cmat, q);
+  if (rc == ORC_OK)
+    rc = (stepper == ORC_STEP_IRK4) ? orc_march_irk4(M, N, p->x, w, p->bc, p->cmat, q)
+                                    : orc_march(M, N, p->x, w, p->bc, p->cmat, q);
   if (rc == ORC_OK) rc = orc_phi_weights(M, N, q, c, phi);
   if (rc == ORC_OK) {
     int last = (p->bc == ORC_BC_DD) ? M - 2 : M - 1;
@@ -437,6 +439,19 @@ int orc_eval(const orc_problem *p, const double *w, double *q_out,
   free(c);
   if (!q_out) free(q);
   return rc;
+}
+
+int orc_eval(const orc_problem *p, const double *w, double *q_out,
+             double *phi, double *r, double *rnorm) {
+  return eval_stepper(p, ORC_STEP_IE, w, q_out, phi, r, rnorm);
+}
+
+/* orc_eval with the march chosen: ORC_STEP_IE or ORC_STEP_IRK4 (the same
+ * pairing, Romberg weights and residual). */
+int orc_eval_stepper(const orc_problem *p, int stepper, const double *w,
+                     double *q_out, double *phi, double *r, double *rnorm) {
+  if (stepper != ORC_STEP_IE && stepper != ORC_STEP_IRK4) return ORC_E_ARG;
+  return eval_stepper(p, stepper, w, q_out, phi, r, rnorm);
 }
 
 /* Mode M: Matlab_files/simple_FEM_1D_transient.m:1-128.  Walls w = 0
@@ -483,6 +498,132 @@ int orc_eval_matlab(int M, const double *x, const double *w,
   free(D3);
   free(q);
   free(rhs);
+  return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* two-stage Gauss-Legendre (IRK4) march                                     */
+/* ------------------------------------------------------------------------ */
+
+/* inverse of a 2x2 block g (row-major) into gi; 0 when singular */
+static int inv2(const double *g, double *gi) {
+  double det = g[0] * g[3] - g[1] * g[2];
+  if (det == 0.0 || !isfinite(det)) return 0;
+  gi[0] = g[3] / det;
+  gi[1] = -g[1] / det;
+  gi[2] = -g[2] / det;
+  gi[3] = g[0] / det;
+  return 1;
+}
+static void mul2(const double *a, const double *b, double *c) {
+  double c0 = a[0] * b[0] + a[1] * b[2], c1 = a[0] * b[1] + a[1] * b[3];
+  double c2 = a[2] * b[0] + a[3] * b[2], c3 = a[2] * b[1] + a[3] * b[3];
+  c[0] = c0; c[1] = c1; c[2] = c2; c[3] = c3;
+}
+
+/* The deal.II path's stepper (DEALII_SCFT/src/scft.cc:671-695,
+ * drivescft.cc:130-146; Matlab_files/IRK4.m:96-120): with D' = B + C, each
+ * step solves for the two stage slopes
+ *   [ A + dt/4 D'               (1/4 - sqrt3/6) dt D' ] [k1]   [-D' q]
+ *   [ (1/4 + sqrt3/6) dt D'     A + dt/4 D'           ] [k2] = [-D' q]
+ * and sets q <- q + dt/2 (k1 + k2).  Dirichlet nodes keep k = 0 (IRK4.m:
+ * 81-95 identity rows with a zero right-hand side), so only the unknowns
+ * enter.  The 2n x 2n system is solved the way it is written: unknown i
+ * carries the pair (k1_i, k2_i), which makes it block tridiagonal with 2x2
+ * blocks, factored once (block Thomas, explicit 2x2 inverses) and reused for
+ * every step.  Same initial condition, step count and slab layout as
+ * orc_march. */
+int orc_march_irk4(int M, int N, const double *x, const double *w, int bc,
+                   int cmat, double *q) {
+  if (M < 3 || N < 1) return ORC_E_ARG;
+  const double dt = 1.0 / N;
+  const double c01 = (0.25 - sqrt(3.0) / 6.0) * dt, c10 = (0.25 + sqrt(3.0) / 6.0) * dt;
+  const int last = (bc == ORC_BC_DD) ? M - 2 : M - 1; /* unknowns 1..last */
+  const int n = last;
+  double *A3 = (double *)malloc(sizeof(double) * 9 * M);
+  double *B3 = A3 + 3 * M, *C3 = A3 + 6 * M;
+  double *blk = (double *)malloc(sizeof(double) * (size_t)n * 3 * 4); /* sub, diag, sup */
+  double *Lm = (double *)malloc(sizeof(double) * (size_t)n * 4);
+  double *Gi = (double *)malloc(sizeof(double) * (size_t)n * 4);
+  double *y = (double *)malloc(sizeof(double) * (size_t)n * 2);
+  double *Dp = (double *)malloc(sizeof(double) * 3 * M);
+  int rc = ORC_OK;
+  orc_assemble(M, x, w, cmat, A3, B3, C3);
+  for (int k = 0; k < 3 * M; k++) Dp[k] = B3[k] + C3[k];
+  for (int j = 0; j < n; j++) {
+    int i = j + 1;
+    for (int e = 0; e < 3; e++) {
+      double a = A3[3 * i + e], d = Dp[3 * i + e];
+      /* columns outside the unknowns are Dirichlet nodes (k = 0) */
+      if ((e == 0 && j == 0) || (e == 2 && j == n - 1)) a = d = 0.0;
+      double *b = blk + ((size_t)j * 3 + e) * 4;
+      b[0] = a + dt / 4.0 * d;
+      b[1] = c01 * d;
+      b[2] = c10 * d;
+      b[3] = a + dt / 4.0 * d;
+    }
+  }
+  /* block factorisation: G_0 = D_0, L_j = S_j G_{j-1}^{-1}, G_j = D_j - L_j U_{j-1} */
+  {
+    double G[4];
+    memcpy(G, blk + 4, sizeof G);
+    if (!inv2(G, Gi)) { rc = ORC_E_PIVOT; goto done; }
+    for (int j = 1; j < n; j++) {
+      const double *S = blk + ((size_t)j * 3 + 0) * 4;
+      const double *Dj = blk + ((size_t)j * 3 + 1) * 4;
+      const double *Up = blk + ((size_t)(j - 1) * 3 + 2) * 4;
+      double T[4];
+      mul2(S, Gi + 4 * (j - 1), Lm + 4 * j);
+      mul2(Lm + 4 * j, Up, T);
+      for (int e = 0; e < 4; e++) G[e] = Dj[e] - T[e];
+      if (!inv2(G, Gi + 4 * j)) { rc = ORC_E_PIVOT; goto done; }
+    }
+  }
+  for (int i = 0; i < M; i++) q[i] = 1.0;
+  q[0] = 0.0;
+  if (bc == ORC_BC_DD) q[M - 1] = 0.0;
+  for (int s = 0; s < N; s++) {
+    const double *qo = q + (size_t)s * M;
+    double *qn = q + (size_t)(s + 1) * M;
+    /* forward: y_j = F_j - L_j y_{j-1}, F_j = (f_j, f_j), f = -D' q */
+    for (int j = 0; j < n; j++) {
+      int i = j + 1;
+      double f = Dp[3 * i + 1] * qo[i] + Dp[3 * i] * qo[i - 1];
+      if (i + 1 < M) f += Dp[3 * i + 2] * qo[i + 1];
+      f = -f;
+      double y0 = f, y1 = f;
+      if (j > 0) {
+        const double *L = Lm + 4 * j;
+        y0 -= L[0] * y[2 * (j - 1)] + L[1] * y[2 * (j - 1) + 1];
+        y1 -= L[2] * y[2 * (j - 1)] + L[3] * y[2 * (j - 1) + 1];
+      }
+      y[2 * j] = y0;
+      y[2 * j + 1] = y1;
+    }
+    /* backward: k_j = G_j^{-1} (y_j - U_j k_{j+1}), stored over y */
+    for (int j = n - 1; j >= 0; j--) {
+      double t0 = y[2 * j], t1 = y[2 * j + 1];
+      if (j < n - 1) {
+        const double *U = blk + ((size_t)j * 3 + 2) * 4;
+        t0 -= U[0] * y[2 * (j + 1)] + U[1] * y[2 * (j + 1) + 1];
+        t1 -= U[2] * y[2 * (j + 1)] + U[3] * y[2 * (j + 1) + 1];
+      }
+      const double *G = Gi + 4 * j;
+      y[2 * j] = G[0] * t0 + G[1] * t1;
+      y[2 * j + 1] = G[2] * t0 + G[3] * t1;
+    }
+    for (int i = 0; i < M; i++) qn[i] = qo[i];
+    for (int j = 0; j < n; j++) qn[j + 1] = qo[j + 1] + 0.5 * dt * (y[2 * j] + y[2 * j + 1]);
+  }
+  for (size_t k = 0; k < (size_t)(N + 1) * M; k++)
+    if (!isfinite(q[k])) { rc = ORC_E_NAN; break; }
+done:
+  free(A3);
+  free(blk);
+  free(Lm);
+  free(Gi);
+  free(y);
+  free(Dp);
   return rc;
 }
 

@@ -28,6 +28,7 @@ extern "C" {
 
 enum { ORC_BC_DD = 0, ORC_BC_DN = 1 };
 enum { ORC_C_ROWSCALED = 0, ORC_C_CONSISTENT = 1 };
+enum { ORC_STEP_IE = 0, ORC_STEP_IRK4 = 1 };
 
 #define ORC_OK 0
 #define ORC_E_ARG 1
@@ -59,6 +60,10 @@ int orc_march(int M, int N, const double *x, const double *w, int bc,
               int cmat, double *q);
 int orc_march_ld(int M, int N, const double *x, const double *w, int bc,
                  int cmat, double *q);
+/* Two-stage Gauss-Legendre (IRK4) march of the deal.II path: the 2x2-block
+ * stage system solved as written (block Thomas), q <- q + dt/2 (k1 + k2). */
+int orc_march_irk4(int M, int N, const double *x, const double *w, int bc,
+                   int cmat, double *q);
 
 /* ---- quadrature (a6) ---- */
 double orc_romint(const double *f, int m, double hh);
@@ -77,6 +82,8 @@ int orc_phi_weights(int M, int N, const double *q, const double *c,
                     double *phi);
 int orc_eval(const orc_problem *p, const double *w, double *q_out,
              double *phi, double *r, double *rnorm);
+int orc_eval_stepper(const orc_problem *p, int stepper, const double *w,
+                     double *q_out, double *phi, double *r, double *rnorm);
 int orc_n_unknowns(int M, int bc);
 int orc_first_unknown(void);
 

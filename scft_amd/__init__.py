@@ -26,6 +26,7 @@ LIB_PATH = os.path.join(HERE, "libscft_hip.so")
 SCFT_OK = 0
 BC_DD, BC_DN = 0, 1
 C_ROWSCALED, C_CONSISTENT = 0, 1
+STEP_IE, STEP_IRK4 = 0, 1
 ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK, ST_FAILED = 1, 2, 4, 8, 16, 32
 MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 L_REF = 3.72374357332160
@@ -44,14 +45,15 @@ class ScftError(RuntimeError):
 
 class Config(C.Structure):
     _fields_ = [("device", C.c_int), ("N", C.c_int), ("L", C.c_double), ("tau", C.c_double),
-                ("bc", C.c_int), ("cmat", C.c_int), ("beta", C.c_double), ("verbose", C.c_int)]
+                ("bc", C.c_int), ("cmat", C.c_int), ("beta", C.c_double), ("verbose", C.c_int),
+                ("stepper", C.c_int)]
 
 
 class BatchDesc(C.Structure):
     _fields_ = [("n_inst", C.c_int), ("M", C.POINTER(C.c_int)), ("N", C.POINTER(C.c_int)),
                 ("xoff", C.POINTER(C.c_longlong)), ("x", C.POINTER(C.c_double)),
                 ("phi_t", C.POINTER(C.c_double)), ("bc", C.c_int), ("cmat", C.c_int),
-                ("tune_file", C.c_char_p)]
+                ("tune_file", C.c_char_p), ("stepper", C.c_int)]
 
 
 class AndersonOpts(C.Structure):
@@ -118,6 +120,7 @@ def _load():
     L.scft_batch_autotune.argtypes = [C.c_void_p, C.c_int]
     L.scft_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), _ip, _ip, _ip, C.c_int]
     L.scft_debug_slabs.argtypes = [C.c_void_p, C.c_int, _dp]
+    L.scft_batch_set_walls.argtypes = [C.c_void_p, _dp, _dp]
     _fp = np.ctypeslib.ndpointer(np.float32, flags="C")
     L.scft_mlp_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _ip, C.c_ulonglong]
     L.scft_mlp_destroy.argtypes = [C.c_void_p]
@@ -162,11 +165,12 @@ def _f64(a):
 
 # ------------------------------------------------------------------ drop-in
 def init(N=2048, L=L_REF, tau=TAU, bc=BC_DD, cmat=C_ROWSCALED, beta=0.1, device=0,
-         verbose=False):
+         verbose=False, stepper=STEP_IE):
     cfg = Config()
     lib.scft_default_config(C.byref(cfg))
     cfg.device, cfg.N, cfg.L, cfg.tau = device, N, L, tau
     cfg.bc, cfg.cmat, cfg.beta, cfg.verbose = bc, cmat, beta, 1 if verbose else 0
+    cfg.stepper = stepper
     check(lib.scft_init(C.byref(cfg)), "scft_init")
 
 
@@ -214,8 +218,9 @@ class Batch:
     list of targets on each instance's unknowns.
     """
 
-    def __init__(self, xs, Ns, phits, bc=BC_DN, cmat=C_ROWSCALED, tune_file=None):
-        self.bc, self.cmat = bc, cmat
+    def __init__(self, xs, Ns, phits, bc=BC_DN, cmat=C_ROWSCALED, tune_file=None,
+                 stepper=STEP_IE):
+        self.bc, self.cmat, self.stepper = bc, cmat, stepper
         self.M = np.array([len(x) for x in xs], dtype=np.int32)
         self.N = np.ascontiguousarray(Ns, dtype=np.int32)
         self.n = self.M - (2 if bc == BC_DD else 1)
@@ -229,7 +234,7 @@ class Batch:
                       xoff.ctypes.data_as(C.POINTER(C.c_longlong)),
                       self._x.ctypes.data_as(C.POINTER(C.c_double)),
                       self._pt.ctypes.data_as(C.POINTER(C.c_double)), bc, cmat,
-                      tune_file.encode() if tune_file else None)
+                      tune_file.encode() if tune_file else None, stepper)
         h = C.c_void_p()
         check(lib.scft_batch_create(C.byref(h), C.byref(d)), "scft_batch_create")
         self.h = h
@@ -337,6 +342,14 @@ class Batch:
         W = np.zeros(16, dtype=np.int32)
         check(lib.scft_batch_tuning(self.h, C.byref(nb), nm, K, W, 16), "scft_batch_tuning")
         return [(int(nm[i]), int(K[i]), int(W[i])) for i in range(nb.value)]
+
+    def set_walls(self, wall0, wallL=None):
+        """Per-instance field values at the walls for the consistent w-mass
+        (default 0)."""
+        w0 = _f64(np.broadcast_to(np.asarray(wall0, dtype=np.float64), (len(self.M),)))
+        wl = _f64(np.broadcast_to(np.asarray(0.0 if wallL is None else wallL, dtype=np.float64),
+                                  (len(self.M),)))
+        check(lib.scft_batch_set_walls(self.h, w0, wl), "scft_batch_set_walls")
 
     def debug_slabs(self, inst=0):
         """Stored slabs q_1..q_{N/2-1} of a single-instance batch (node order)."""

@@ -18,7 +18,8 @@ LIB = os.path.join(HERE, "libscft_hip.so")
 OBJDIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
-SOURCES = ["host_math.cpp", "march.hip", "anderson.hip", "mlp.hip", "capi.hip", "broyden.cpp"]
+SOURCES = ["host_math.cpp", "march.hip", "irk4.hip", "anderson.hip", "mlp.hip", "capi.hip",
+           "broyden.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           f"--offload-arch={ARCH}",

@@ -68,7 +68,11 @@ struct Bucket {
   int stream = 0;             // which of the batch's streams launches it
 };
 
-int default_K(int cap) { return std::min(cap / 64, 16); }
+// rows per lane of a new bucket: implicit Euler keeps ~8 doubles per row in
+// registers, IRK4 ~11 (its variants stop at K = 8)
+int default_K(int cap, int stepper) {
+  return std::min(cap / 64, stepper == SCFT_STEP_IRK4 ? 8 : 16);
+}
 
 int cap_for(int n) {
   int c = 64;
@@ -82,6 +86,7 @@ struct scft_batch {
   int device = 0;
   int n_inst = 0;
   int bc = SCFT_BC_DD, cmat = SCFT_C_ROWSCALED;
+  int stepper = SCFT_STEP_IE;
   std::vector<InstDesc> desc;
   std::vector<long long> uoff;
   long long total_u = 0, total_x = 0, total_pw = 0, total_h = 0;
@@ -113,6 +118,21 @@ struct scft_batch {
   int last_sweeps = 0;
 };
 
+// The march of the batch's stepper: implicit Euler (march.hip) or IRK4
+// (irk4.hip); both take the same arguments, slabs and buckets.
+static bool variant_exists(const scft_batch* b, int K, int W) {
+  return b->stepper == SCFT_STEP_IRK4 ? irk4_variant_exists(K, W) : march_variant_exists(K, W);
+}
+static hipError_t launch_eval(const scft_batch* b, int K, int W, const MarchArgs& a, int grid,
+                              hipStream_t s) {
+  return b->stepper == SCFT_STEP_IRK4 ? launch_irk4(K, W, a, grid, s)
+                                      : launch_march(K, W, a, grid, s);
+}
+static int eval_max_grid(const scft_batch* b, int K, int W, int half_max) {
+  return b->stepper == SCFT_STEP_IRK4 ? irk4_max_grid(K, W, half_max)
+                                      : march_max_grid(K, W, half_max);
+}
+
 // Algorithmic HBM bytes of one evaluation: q-slabs 1..N/2-1 written once and
 // read back once, plus the O(n) inputs/outputs (x, w, phi_t in; phi, r out).
 static double eval_bytes(int n, int N) {
@@ -128,12 +148,17 @@ static int size_scratch(scft_batch* b) {
   for (size_t i = 0; i < b->buckets.size(); i++) {
     Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
-    if (!march_variant_exists(bk.K, bk.W)) {
+    if (!variant_exists(b, bk.K, bk.W)) {
       snprintf(g_errmsg, sizeof g_errmsg, "no march variant K=%d W=%d", bk.K, bk.W);
       return SCFT_E_ARG;
     }
     const int P = 64 * bk.W;
-    const int mg = march_max_grid(bk.K, bk.W, bk.Nmax / 2);
+    const int mg = eval_max_grid(b, bk.K, bk.W, bk.Nmax / 2);
+    if (mg < 1) {
+      snprintf(g_errmsg, sizeof g_errmsg, "march variant K=%d W=%d does not fit a CU", bk.K,
+               bk.W);
+      return SCFT_E_ARG;
+    }
     bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), mg));
     bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2)) * P * bk.K;
     bk.slab_base = need;
@@ -166,7 +191,7 @@ static void read_tune_file(scft_batch* b) {
   int cap, K, W;
   while (f >> cap >> K >> W) {
     for (auto& bk : b->buckets)
-      if (bk.cap == cap && 64 * K * W >= cap && march_variant_exists(K, W)) {
+      if (bk.cap == cap && 64 * K * W >= cap && variant_exists(b, K, W)) {
         bk.K = K;
         bk.W = W;
       }
@@ -218,7 +243,7 @@ static int enqueue_sweep(scft_batch* b) {
   for (int i = nb - 1; i >= 0; i--) {
     const Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
-    HIPCHK(launch_march(bk.K, bk.W, march_args(b, bk, i), bk.grid, stream_of(b, bk.stream)));
+    HIPCHK(launch_eval(b, bk.K, bk.W, march_args(b, bk, i), bk.grid, stream_of(b, bk.stream)));
   }
   for (int s = 1; s <= scft_batch::kSide; s++)
     if (used[s]) {
@@ -235,11 +260,13 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
     return SCFT_E_ARG;
   if (dsc->bc != SCFT_BC_DD && dsc->bc != SCFT_BC_DN) return SCFT_E_ARG;
   if (dsc->cmat != SCFT_C_ROWSCALED && dsc->cmat != SCFT_C_CONSISTENT) return SCFT_E_ARG;
+  if (dsc->stepper != SCFT_STEP_IE && dsc->stepper != SCFT_STEP_IRK4) return SCFT_E_ARG;
   *out = nullptr;
   auto* b = new scft_batch();
   b->n_inst = dsc->n_inst;
   b->bc = dsc->bc;
   b->cmat = dsc->cmat;
+  b->stepper = dsc->stepper;
   if (dsc->tune_file) b->tune_file = dsc->tune_file;
   if (hipGetDevice(&b->device) != hipSuccess) {
     delete b;
@@ -256,7 +283,9 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
     const int M = dsc->M[k], N = dsc->N[k];
     // N: a power of two (Romberg) in [16, 16384] (the N/2 + 1 pairing weights
     // of a bucket are held in LDS)
-    if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > 4096 || N < 16 || N > 16384 ||
+    // IRK4 keeps more per-row state in registers: up to 2048 unknowns
+    const int n_max = (b->stepper == SCFT_STEP_IRK4) ? 2048 : 4096;
+    if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > n_max || N < 16 || N > 16384 ||
         (N & (N - 1)) != 0) {
       snprintf(g_errmsg, sizeof g_errmsg, "instance %d: M=%d N=%d unsupported", k, M, N);
       delete b;
@@ -305,7 +334,7 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
   for (int cap = 64; cap <= 4096; cap *= 2) {
     Bucket bk;
     bk.cap = cap;
-    bk.K = default_K(cap);
+    bk.K = default_K(cap, b->stepper);
     bk.W = cap / 64 / bk.K;
     b->buckets.push_back(bk);
   }
@@ -404,6 +433,19 @@ extern "C" int scft_batch_layout(const scft_batch* b, long long* uoff, long long
   return SCFT_OK;
 }
 
+extern "C" int scft_batch_set_walls(scft_batch* b, const double* wall0, const double* wallL) {
+  if (!b || !wall0 || !wallL) return SCFT_E_ARG;
+  for (int k = 0; k < b->n_inst; k++) {
+    if (!std::isfinite(wall0[k]) || !std::isfinite(wallL[k])) return SCFT_E_ARG;
+    b->desc[k].wall0 = wall0[k];
+    b->desc[k].wallL = wallL[k];
+  }
+  HIPCHK(hipMemcpyAsync(b->d_desc, b->desc.data(), sizeof(InstDesc) * b->n_inst,
+                        hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCFT_OK;
+}
+
 // Replace the target of every instance (host array on unknowns).
 static int batch_set_target(scft_batch* b, const double* phit) {
   HIPCHK(hipMemcpyAsync(b->d_phit, phit, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
@@ -479,12 +521,12 @@ extern "C" int scft_batch_time_buckets(scft_batch* b, const double* w, int reps,
     if (bk.items.empty()) continue;
     const MarchArgs a = march_args(b, bk, i);
     HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-    HIPCHK(launch_march(bk.K, bk.W, a, bk.grid, b->stream));  // warm-up
+    HIPCHK(launch_eval(b, bk.K, bk.W, a, bk.grid, b->stream));  // warm-up
     HIPCHK(hipEventRecord(e0, b->stream));
     for (int r = 0; r < reps; r++) {
       // the counter reset is a ~2 us kernel inside the timed region
       HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-      HIPCHK(launch_march(bk.K, bk.W, a, bk.grid, b->stream));
+      HIPCHK(launch_eval(b, bk.K, bk.W, a, bk.grid, b->stream));
     }
     HIPCHK(hipEventRecord(e1, b->stream));
     HIPCHK(hipEventSynchronize(e1));
@@ -524,17 +566,17 @@ extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
     for (int K : Ks)
       for (int W : Wsz) {
         if (64 * K * W < bk.nmax || 64 * K * W > 2 * bk.cap) continue;
-        if (!march_variant_exists(K, W)) continue;
+        if (!variant_exists(b, K, W) || eval_max_grid(b, K, W, bk.Nmax / 2) < 1) continue;
         bk.K = K;
         bk.W = W;
         SCFT_TRY(size_scratch(b));
         MarchArgs a = march_args(b, bk, bi);
         HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-        HIPCHK(launch_march(K, W, a, bk.grid, b->stream));  // warm-up
+        HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));  // warm-up
         HIPCHK(hipEventRecord(e0, b->stream));
         for (int r = 0; r < reps; r++) {
           HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-          HIPCHK(launch_march(K, W, a, bk.grid, b->stream));
+          HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));
         }
         HIPCHK(hipEventRecord(e1, b->stream));
         HIPCHK(hipEventSynchronize(e1));
@@ -834,12 +876,14 @@ extern "C" void scft_default_config(scft_config* c) {
   c->cmat = SCFT_C_ROWSCALED;
   c->beta = 0.1;
   c->verbose = 0;
+  c->stepper = SCFT_STEP_IE;
 }
 
 static int init_locked(const scft_config* cfg) {
   if (cfg) {
     if (cfg->N < 16 || (cfg->N & (cfg->N - 1)) != 0 || !(cfg->L > 0.0) ||
-        (cfg->bc != SCFT_BC_DD && cfg->bc != SCFT_BC_DN))
+        (cfg->bc != SCFT_BC_DD && cfg->bc != SCFT_BC_DN) ||
+        (cfg->stepper != SCFT_STEP_IE && cfg->stepper != SCFT_STEP_IRK4))
       return SCFT_E_ARG;
     g_cfg = *cfg;
   } else {
@@ -919,6 +963,7 @@ static int dropin_eval(int n, const double* w, double* r) {
     d.phi_t = g_ptu.data();
     d.bc = g_cfg.bc;
     d.cmat = g_cfg.cmat;
+    d.stepper = g_cfg.stepper;
     d.tune_file = nullptr;
     SCFT_TRY(scft_batch_create(&g_one, &d));
     g_one_n = n;
@@ -935,6 +980,13 @@ static int dropin_eval(int n, const double* w, double* r) {
   }
   for (int u = 0; u < n; u++) g_ptu[u] = g_ptall[u + 1];
   SCFT_TRY(batch_set_target(g_one, g_ptu.data()));
+  // the consistent w-mass reads the field at the walls: extrapolated from the
+  // interior as the deal.II evaluator does (DD; needs 2+ interior nodes)
+  if (g_cfg.cmat == SCFT_C_CONSISTENT && dd && M >= 4) {
+    double w0 = 0.0, wl = 0.0;
+    if (!wall_values_natural(M, g_x.data(), w, &w0, &wl)) return SCFT_E_ARG;
+    SCFT_TRY(scft_batch_set_walls(g_one, &w0, &wl));
+  }
   double rn = 0.0;
   unsigned st = 0;
   SCFT_TRY(batch_eval_impl(g_one, w, nullptr, r, &rn, &st));

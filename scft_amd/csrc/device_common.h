@@ -1,0 +1,111 @@
+// device_common.h -- lane moves and double-double arithmetic shared by the
+// propagator marches (march.hip: implicit Euler, irk4.hip: two-stage Gauss).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace scft {
+
+__device__ __forceinline__ void wg_barrier() { __syncthreads(); }
+
+// DPP lane moves on a double (two 32-bit halves).  CTRL is a gfx9 DPP
+// control: 0x110 + s = row_shr:s (lane p reads p - s of its 16-lane row),
+// 0x100 + s = row_shl:s (reads p + s).  Lanes whose source leaves the row
+// read 0 (bound_ctrl); every use multiplies them by a zero weight.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+template <int S>
+__device__ __forceinline__ double row_shr(double v) { return dpp_d<0x110 + S>(v); }
+template <int S>
+__device__ __forceinline__ double row_shl(double v) { return dpp_d<0x100 + S>(v); }
+
+// value of `v` in lane `l`, as a wave-uniform scalar
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), l);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), l);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
+// ---- double-double arithmetic (hi + lo, |lo| <= ulp(hi)/2) for the pivot
+// scan.  The pivots are a projective product of 2x2 matrices whose two
+// eigenvalues nearly coincide when diffusion dominates (dt/h^2 >> 1); in FP64
+// the lane-entry pivots come out ~1e-14 off, and D's conditioning turns that
+// into ~1e-11 per step in q.  Carrying the scan in double-double (once per
+// evaluation) brings the entry pivots back to FP64 accuracy, so the computed
+// LU is as backward-stable as a sequential Thomas factorisation.  Every
+// routine below turns FMA contraction off: fusing a product into a two-sum
+// (s = fma(a, b, c), then s - a*b as another fma) breaks the error-free
+// transformations, and the device compiler does exactly that by default.
+struct dd {
+  double h, l;
+};
+__device__ __forceinline__ dd dd_from(double a) { return {a, 0.0}; }
+__device__ __forceinline__ dd two_sum(double a, double b) {
+#pragma clang fp contract(off)
+  const double s = a + b;
+  const double bb = s - a;
+  const double e = (a - (s - bb)) + (b - bb);
+  return {s, e};
+}
+__device__ __forceinline__ dd quick_two_sum(double a, double b) {
+#pragma clang fp contract(off)
+  const double s = a + b;
+  return {s, b - (s - a)};
+}
+__device__ __forceinline__ dd two_prod(double a, double b) {
+#pragma clang fp contract(off)
+  const double p = a * b;
+  return {p, fma(a, b, -p)};
+}
+__device__ __forceinline__ dd dd_add(dd a, dd b) {
+#pragma clang fp contract(off)
+  dd s = two_sum(a.h, b.h);
+  dd t = two_sum(a.l, b.l);
+  s.l += t.h;
+  s = quick_two_sum(s.h, s.l);
+  s.l += t.l;
+  return quick_two_sum(s.h, s.l);
+}
+__device__ __forceinline__ dd dd_mul(dd a, dd b) {
+#pragma clang fp contract(off)
+  dd p = two_prod(a.h, b.h);
+  p.l += a.h * b.l + a.l * b.h;
+  return quick_two_sum(p.h, p.l);
+}
+__device__ __forceinline__ dd dd_scale(dd a, double s) { return {a.h * s, a.l * s}; }  // s = 2^k
+__device__ __forceinline__ double dd_div_to_double(dd a, dd b) {
+#pragma clang fp contract(off)
+  const double q1 = a.h / b.h;
+  const dd r = dd_add(a, dd_mul(b, dd_from(-q1)));
+  return q1 + r.h / b.h;
+}
+
+// 2x2 double-double product R = X * Y, rescaled by a power of two so the
+// largest entry is in [1, 2): the pivot is a ratio of entries, so scaling is
+// harmless and long products neither overflow nor underflow.
+__device__ __forceinline__ void mat_mul_norm(const dd X[4], const dd Y[4], dd R[4]) {
+#pragma clang fp contract(off)
+  const dd r0 = dd_add(dd_mul(X[0], Y[0]), dd_mul(X[1], Y[2]));
+  const dd r1 = dd_add(dd_mul(X[0], Y[1]), dd_mul(X[1], Y[3]));
+  const dd r2 = dd_add(dd_mul(X[2], Y[0]), dd_mul(X[3], Y[2]));
+  const dd r3 = dd_add(dd_mul(X[2], Y[1]), dd_mul(X[3], Y[3]));
+  const double mx = fmax(fmax(fabs(r0.h), fabs(r1.h)), fmax(fabs(r2.h), fabs(r3.h)));
+  const double sc = (mx > 0.0 && isfinite(mx)) ? ldexp(1.0, -ilogb(mx)) : 1.0;
+  R[0] = dd_scale(r0, sc);
+  R[1] = dd_scale(r1, sc);
+  R[2] = dd_scale(r2, sc);
+  R[3] = dd_scale(r3, sc);
+}
+
+__device__ __forceinline__ dd shfl_up_dd(dd v, int s) {
+  return {__shfl_up(v.h, s), __shfl_up(v.l, s)};
+}
+
+}  // namespace scft

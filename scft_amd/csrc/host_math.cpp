@@ -136,6 +136,47 @@ bool spline_not_a_knot(int n, const double* x, const double* y, std::vector<doub
   return true;
 }
 
+// Second derivatives of the natural cubic spline (M_0 = M_{n-1} = 0, the
+// reference's m = 0 case, spline_chen.c:60-65): the interior rows alone, a
+// diagonally dominant tridiagonal system solved by Thomas.
+bool spline_natural(int n, const double* x, const double* y, std::vector<double>& M) {
+  if (n < 2) return false;
+  for (int i = 0; i + 1 < n; i++)
+    if (!(x[i + 1] > x[i])) return false;
+  M.assign(n, 0.0);
+  const int k = n - 2;
+  if (k < 1) return true;
+  std::vector<double> lo(k), di(k), up(k), rhs(k);
+  for (int i = 1; i <= n - 2; i++) {
+    const double hl = x[i] - x[i - 1], hr = x[i + 1] - x[i];
+    lo[i - 1] = hl / 6.0;
+    di[i - 1] = (hl + hr) / 3.0;
+    up[i - 1] = hr / 6.0;
+    rhs[i - 1] = (y[i + 1] - y[i]) / hr - (y[i] - y[i - 1]) / hl;
+  }
+  for (int i = 1; i < k; i++) {
+    const double w = lo[i] / di[i - 1];
+    di[i] -= w * up[i - 1];
+    rhs[i] -= w * rhs[i - 1];
+  }
+  M[k] = rhs[k - 1] / di[k - 1];
+  for (int i = k - 2; i >= 0; i--) M[i + 1] = (rhs[i] - up[i] * M[i + 2]) / di[i];
+  return true;
+}
+
+// Field at the walls as the deal.II evaluator sets it before assembling
+// (DEALII_SCFT/src/scft.cc:454-480): the natural spline through the interior
+// nodes 1..M-2, evaluated (extrapolated) at x_0 and x_{M-1}.
+bool wall_values_natural(int M, const double* x, const double* w_interior, double* wall0,
+                         double* wallL) {
+  if (M < 4) return false;
+  std::vector<double> Mi;
+  if (!spline_natural(M - 2, x + 1, w_interior, Mi)) return false;
+  *wall0 = spline_eval(M - 2, x + 1, w_interior, Mi.data(), x[0]);
+  *wallL = spline_eval(M - 2, x + 1, w_interior, Mi.data(), x[M - 1]);
+  return true;
+}
+
 // Value at xp of the spline (x, y, M): bisection for the bracketing
 // interval (the end intervals outside the table) and the cubic in the
 // form of spline_chen.c:88-100.

@@ -10,6 +10,9 @@ void target_position(int M, const double* x, double tau, bool mirror, double* ph
 void mesh_uniform(int M, double L, double* x);
 bool spline_not_a_knot(int n, const double* x, const double* y, std::vector<double>& M);
 double spline_eval(int n, const double* x, const double* y, const double* M, double xp);
+bool spline_natural(int n, const double* x, const double* y, std::vector<double>& M);
+bool wall_values_natural(int M, const double* x, const double* w_interior, double* wall0,
+                         double* wallL);
 std::vector<double> refine_by_gradient(int M, const double* x, const double* w_nodes,
                                        double factor, bool wall_right);
 bool free_energy(int M, const double* x, const double* w_nodes, double L, double tau, bool mirror,

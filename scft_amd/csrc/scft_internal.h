@@ -91,6 +91,10 @@ hipError_t launch_march(int K, int W, const MarchArgs& a, int grid,
 // Largest grid of co-resident workgroups for a variant (persistent launch).
 int march_max_grid(int K, int W, int half_max);
 bool march_variant_exists(int K, int W);
+// The same for the two-stage Gauss (IRK4) march (irk4.hip).
+hipError_t launch_irk4(int K, int W, const MarchArgs& a, int grid, hipStream_t s);
+int irk4_max_grid(int K, int W, int half_max);  // 0: the workgroup does not fit
+bool irk4_variant_exists(int K, int W);
 hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_reset(int* counters, int n, hipStream_t s);
 constexpr int MIX_MAX_HISTORY = 12;
