@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <sstream>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -63,6 +64,7 @@ struct Bucket {
   std::vector<int> items;  // instance ids, largest cost first
   int* d_items = nullptr;
   int grid = 0;
+  int wpc = 0;  // workgroups per CU of the persistent grid (0: as many as fit)
   long long slab_stride = 0;  // doubles per workgroup
   long long slab_base = 0;    // this bucket's region in the scratch (doubles)
   int stream = 0;             // which of the batch's streams launches it
@@ -87,6 +89,7 @@ struct scft_batch {
   int n_inst = 0;
   int bc = SCFT_BC_DD, cmat = SCFT_C_ROWSCALED;
   int stepper = SCFT_STEP_IE;
+  int cus = 1;  // compute units of the device
   std::vector<InstDesc> desc;
   std::vector<long long> uoff;
   long long total_u = 0, total_x = 0, total_pw = 0, total_h = 0;
@@ -159,7 +162,10 @@ static int size_scratch(scft_batch* b) {
                bk.W);
       return SCFT_E_ARG;
     }
-    bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), mg));
+    // fewer co-resident workgroups per CU can finish a bucket sooner: its
+    // longest instances set the makespan, and they step faster alone
+    const int cap_grid = bk.wpc > 0 ? std::min(mg, bk.wpc * b->cus) : mg;
+    bk.grid = std::max(1, std::min<int>(static_cast<int>(bk.items.size()), cap_grid));
     bk.slab_stride = static_cast<long long>(std::max(1, bk.Nmax / 2)) * P * bk.K;
     bk.slab_base = need;
     need += bk.slab_stride * bk.grid;
@@ -188,12 +194,17 @@ static int size_scratch(scft_batch* b) {
 static void read_tune_file(scft_batch* b) {
   if (b->tune_file.empty()) return;
   std::ifstream f(b->tune_file);
-  int cap, K, W;
-  while (f >> cap >> K >> W) {
+  std::string line;
+  while (std::getline(f, line)) {  // "cap K W [workgroups per CU]"
+    std::istringstream ls(line);
+    int cap, K, W, wpc = 0;
+    if (!(ls >> cap >> K >> W)) continue;
+    if (!(ls >> wpc) || wpc < 0) wpc = 0;
     for (auto& bk : b->buckets)
       if (bk.cap == cap && 64 * K * W >= cap && variant_exists(b, K, W)) {
         bk.K = K;
         bk.W = W;
+        bk.wpc = wpc;
       }
   }
 }
@@ -202,7 +213,7 @@ static void write_tune_file(const scft_batch* b) {
   if (b->tune_file.empty()) return;
   std::ofstream f(b->tune_file);
   for (auto& bk : b->buckets)
-    if (!bk.items.empty()) f << bk.cap << " " << bk.K << " " << bk.W << "\n";
+    if (!bk.items.empty()) f << bk.cap << " " << bk.K << " " << bk.W << " " << bk.wpc << "\n";
 }
 
 static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_idx) {
@@ -268,10 +279,13 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
   b->cmat = dsc->cmat;
   b->stepper = dsc->stepper;
   if (dsc->tune_file) b->tune_file = dsc->tune_file;
-  if (hipGetDevice(&b->device) != hipSuccess) {
+  if (hipGetDevice(&b->device) != hipSuccess ||
+      hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device) !=
+          hipSuccess) {
     delete b;
     return SCFT_E_HIP;
   }
+  b->cus = std::max(1, b->cus);
   // descriptors, offsets, pairing weights per distinct N
   std::vector<int> Ns;
   std::vector<std::vector<double>> pws;
@@ -562,34 +576,42 @@ extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
     Bucket& bk = b->buckets[bi];
     if (bk.items.empty()) continue;
     float best = 1e30f;
-    int bestK = bk.K, bestW = bk.W;
+    int bestK = bk.K, bestW = bk.W, bestP = bk.wpc;
     for (int K : Ks)
       for (int W : Wsz) {
         if (64 * K * W < bk.nmax || 64 * K * W > 2 * bk.cap) continue;
-        if (!variant_exists(b, K, W) || eval_max_grid(b, K, W, bk.Nmax / 2) < 1) continue;
-        bk.K = K;
-        bk.W = W;
-        SCFT_TRY(size_scratch(b));
-        MarchArgs a = march_args(b, bk, bi);
-        HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-        HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));  // warm-up
-        HIPCHK(hipEventRecord(e0, b->stream));
-        for (int r = 0; r < reps; r++) {
+        if (!variant_exists(b, K, W)) continue;
+        const int mg = eval_max_grid(b, K, W, bk.Nmax / 2);
+        if (mg < 1) continue;
+        const int occ = std::max(1, mg / b->cus);
+        for (int wpc = 1; wpc <= occ; wpc++) {
+          bk.K = K;
+          bk.W = W;
+          bk.wpc = (wpc == occ) ? 0 : wpc;
+          SCFT_TRY(size_scratch(b));
+          MarchArgs a = march_args(b, bk, bi);
           HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-          HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));
-        }
-        HIPCHK(hipEventRecord(e1, b->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-          best = ms;
-          bestK = K;
-          bestW = W;
+          HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));  // warm-up
+          HIPCHK(hipEventRecord(e0, b->stream));
+          for (int r = 0; r < reps; r++) {
+            HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+            HIPCHK(launch_eval(b, K, W, a, bk.grid, b->stream));
+          }
+          HIPCHK(hipEventRecord(e1, b->stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          if (ms < best) {
+            best = ms;
+            bestK = K;
+            bestW = W;
+            bestP = bk.wpc;
+          }
         }
       }
     bk.K = bestK;
     bk.W = bestW;
+    bk.wpc = bestP;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

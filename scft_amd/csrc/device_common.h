@@ -33,6 +33,25 @@ __device__ __forceinline__ double readlane_d(double v, int l) {
                               static_cast<unsigned int>(lo));
 }
 
+// Slab scratch as a raw buffer: a scalar resource (base, size) plus a
+// wave-uniform byte offset and a per-lane byte offset, one instruction per
+// access with no per-lane address arithmetic.  Accesses past `bytes` read 0
+// and are dropped by the hardware.  aux = 2 sets the non-temporal hint: each
+// slab element is written once and read once, steps apart.  The third
+// descriptor word is the raw-buffer format word of the GFX9 family.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slab_rsrc(double* base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, 0, static_cast<int>(bytes), 0x00020000);
+}
+__device__ __forceinline__ void slab_store(double v, __amdgpu_buffer_rsrc_t r, unsigned lane_off,
+                                           unsigned uni_off) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, lane_off, uni_off, 2);
+}
+__device__ __forceinline__ double slab_load(__amdgpu_buffer_rsrc_t r, unsigned lane_off,
+                                            unsigned uni_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 2));
+}
+
 // ---- double-double arithmetic (hi + lo, |lo| <= ulp(hi)/2) for the pivot
 // scan.  The pivots are a projective product of 2x2 matrices whose two
 // eigenvalues nearly coincide when diffusion dominates (dt/h^2 >> 1); in FP64

@@ -117,6 +117,8 @@ struct Irk4Shared {
   cd Gw[W];          // backward wave transfer products (per instance)
   cd cx[W];          // last super-diagonal of S in each wave (setup)
   cd gx[W];          // last gamma of each wave (setup)
+  cd fz[W][W];       // forward fold multipliers of each wave (per instance)
+  cd fx[W][W];       // backward fold multipliers of each wave (per instance)
   double Mx[W][4][4];  // wave products of the pivot matrices (setup)
   double red[W];     // norm reduction
   unsigned bad[W];
@@ -351,6 +353,18 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
     wg_barrier();
     const cd gx_prev = sh.gx[wave > 0 ? wave - 1 : 0];
     gl = c_sel(lane == 0, c_sel(wave > 0, gx_prev, c0), gl);
+    // fold multipliers (as in march.hip; complex, so kept in LDS rather than
+    // scalar registers): Z = sum_{v < wave} fz[v] Bw[v], X = sum_{v > wave} fx[v] Cw[v]
+    cd p = c1v;
+    for (int v = W - 1; v >= 0; v--) {
+      sh.fz[wave][v] = c_sel(v < wave, p, c0);
+      p = c_sel(v < wave, c_mul(p, sh.Aw[v]), p);
+    }
+    p = c1v;
+    for (int v = 0; v < W; v++) {
+      sh.fx[wave][v] = c_sel(v > wave, p, c0);
+      p = c_sel(v > wave, c_mul(p, sh.Gw[v]), p);
+    }
   } else {
     gl = c_sel(lane == 0, c0, gl);
   }
@@ -395,7 +409,8 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
       sh.Bw[wave] = c_fma(Arow3, k3, c_readlane(B, 63));
       wg_barrier();
       cd Z = c0;
-      for (int v = 0; v < wave; v++) Z = c_fma(sh.Aw[v], Z, sh.Bw[v]);
+#pragma unroll
+      for (int v = 0; v + 1 < W; v++) Z = c_fma(sh.fz[wave][v], sh.Bw[v], Z);
       zin = c_fma(Apre_m1, Z, zin);
     }
     {
@@ -424,7 +439,8 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
       sh.Cw[wave] = c_fma(Grow0, e0, c_readlane(Cc, 0));
       wg_barrier();
       cd X = c0;
-      for (int v = W - 1; v > wave; v--) X = c_fma(sh.Gw[v], X, sh.Cw[v]);
+#pragma unroll
+      for (int v = 1; v < W; v++) X = c_fma(sh.fx[wave][v], sh.Cw[v], X);
       xin = c_fma(Gsuf_p1, X, xin);
     }
     cd xx = xin;
@@ -439,21 +455,21 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
 
   // slabs, pairing and residual: the layout and order of march.hip
   const bool lane_live = u0 < n;
-  const int lane_src = lane_live ? tid : 0;
-  {
-    double* __restrict__ dst = slab + tid;
+  const __amdgpu_buffer_rsrc_t srsrc =
+      slab_rsrc(slab, static_cast<unsigned>(a.slab_stride * sizeof(double)));
+  const unsigned off_dst = static_cast<unsigned>(tid) * 8u;
+  const unsigned off_src = lane_live ? off_dst : 0u;
+  auto store_slab = [&](const int j) __attribute__((always_inline)) {
     if (lane_live) {
 #pragma unroll
-      for (int t = 0; t < K; t++) dst[t * P] = q[t];
+      for (int t = 0; t < K; t++)
+        slab_store(q[t], srsrc, off_dst, static_cast<unsigned>((j * (P * K) + t * P) * 8));
     }
-    for (int j = 1; j < half; j++) {
-      advance();
-      dst += P * K;
-      if (lane_live) {
-#pragma unroll
-        for (int t = 0; t < K; t++) dst[t * P] = q[t];
-      }
-    }
+  };
+  store_slab(0);
+  for (int j = 1; j < half; j++) {
+    advance();
+    store_slab(j);
   }
   advance();
   {
@@ -464,9 +480,9 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
   constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
   double ring[PD][K];
   auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
-    const double* __restrict__ src = slab + (long long)max(s, 0) * (P * K) + lane_src;
 #pragma unroll
-    for (int t = 0; t < K; t++) dstv[t] = src[t * P];
+    for (int t = 0; t < K; t++)
+      dstv[t] = slab_load(srsrc, off_src, static_cast<unsigned>((max(s, 0) * (P * K) + t * P) * 8));
   };
 #pragma unroll
   for (int e = 0; e < PD; e++) fetch(half - 1 - e, ring[e]);

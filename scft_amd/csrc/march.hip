@@ -272,6 +272,13 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   const double Apre_m1 = ((rl > 0) ? Ashift : 1.0) * Abefore;
   const double Gsuf_p1 = ((rl < 15) ? Gshift : 1.0) * Gafter;
   double gl = __shfl_up(ga[K - 1], 1);  // gamma of the row left of me
+  // Fold multipliers.  The forward carry into this wave from the waves before
+  // it is Z = sum_{v < wave} fz[v] Bw[v] with fz[v] = prod_{v < u < wave} Aw[u];
+  // the backward carry from the waves after it is X = sum_{v > wave} fx[v] Cw[v]
+  // with fx[v] = prod_{wave < u < v} Gw[u].  Fixed per instance (0 where the
+  // wave does not read) and held as wave-uniform values, so a step reads all
+  // wave totals at once: one LDS round trip instead of one per earlier wave.
+  [[maybe_unused]] double fz[W], fx[W];
   if constexpr (W > 1) {
     sh.Aw[wave] = Arow0 * Arow1 * Arow2 * Arow3;
     sh.gx[wave] = readlane_d(ga[K - 1], 63);
@@ -279,6 +286,18 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     wg_barrier();
     const double gx_prev = (wave > 0) ? sh.gx[wave - 1] : 0.0;
     gl = (lane == 0) ? gx_prev : gl;
+    double p = 1.0;
+#pragma unroll
+    for (int v = W - 1; v >= 0; v--) {
+      fz[v] = readlane_d((v < wave) ? p : 0.0, 0);
+      p = (v < wave) ? p * sh.Aw[v] : p;
+    }
+    p = 1.0;
+#pragma unroll
+    for (int v = 0; v < W; v++) {
+      fx[v] = readlane_d((v > wave) ? p : 0.0, 0);
+      p = (v > wave) ? p * sh.Gw[v] : p;
+    }
   } else {
     gl = (lane == 0) ? 0.0 : gl;
   }
@@ -322,9 +341,13 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     if constexpr (W > 1) {
       sh.Bw[wave] = fma(Arow3, c3, readlane_d(B, 63));
       wg_barrier();
-      double Z = 0.0;
-      for (int v = 0; v < wave; v++) Z = fma(sh.Aw[v], Z, sh.Bw[v]);
-      zin = fma(Apre_m1, Z, zin);
+      double Z0 = 0.0, Z1 = 0.0;
+#pragma unroll
+      for (int v = 0; v + 1 < W; v += 2) {
+        Z0 = fma(fz[v], sh.Bw[v], Z0);
+        if (v + 2 < W) Z1 = fma(fz[v + 1], sh.Bw[v + 1], Z1);
+      }
+      zin = fma(Apre_m1, Z0 + Z1, zin);
     }
     {
       double z = zin;
@@ -352,9 +375,13 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     if constexpr (W > 1) {
       sh.Cw[wave] = fma(Grow0, e0, readlane_d(Cc, 0));
       wg_barrier();
-      double X = 0.0;
-      for (int v = W - 1; v > wave; v--) X = fma(sh.Gw[v], X, sh.Cw[v]);
-      xin = fma(Gsuf_p1, X, xin);
+      double X0 = 0.0, X1 = 0.0;
+#pragma unroll
+      for (int v = 1; v < W; v += 2) {
+        X0 = fma(fx[v], sh.Cw[v], X0);
+        if (v + 1 < W) X1 = fma(fx[v + 1], sh.Cw[v + 1], X1);
+      }
+      xin = fma(Gsuf_p1, X0 + X1, xin);
     }
     {
       double xx = xin;
@@ -372,24 +399,26 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   // condition) sits at s * P * K + t * P + tid.  Lanes whose rows are all
   // padding (u0 >= n) store nothing (one exec mask for the whole loop) and
   // load from lane 0's address, so a wave's dead lanes share one cache line;
-  // what they load only ever multiplies their q = 0.
+  // what they load only ever multiplies their q = 0.  The region is accessed
+  // as a raw buffer (slab_rsrc): the step and element select a scalar
+  // offset, the lane a fixed 32-bit offset.
   // First half: nothing here waits on memory.
   const bool lane_live = u0 < n;
-  const int lane_src = lane_live ? tid : 0;
-  {
-    double* __restrict__ dst = slab + tid;
+  const __amdgpu_buffer_rsrc_t srsrc =
+      slab_rsrc(slab, static_cast<unsigned>(a.slab_stride * sizeof(double)));
+  const unsigned off_dst = static_cast<unsigned>(tid) * 8u;
+  const unsigned off_src = lane_live ? off_dst : 0u;
+  auto store_slab = [&](const int j) __attribute__((always_inline)) {
     if (lane_live) {
 #pragma unroll
-      for (int t = 0; t < K; t++) dst[t * P] = q[t];
+      for (int t = 0; t < K; t++)
+        slab_store(q[t], srsrc, off_dst, static_cast<unsigned>((j * (P * K) + t * P) * 8));
     }
-    for (int j = 1; j < half; j++) {
-      advance();
-      dst += P * K;
-      if (lane_live) {
-#pragma unroll
-        for (int t = 0; t < K; t++) dst[t * P] = q[t];
-      }
-    }
+  };
+  store_slab(0);
+  for (int j = 1; j < half; j++) {
+    advance();
+    store_slab(j);
   }
   // step N/2 pairs with itself
   advance();
@@ -406,9 +435,9 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
   double ring[PD][K];
   auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
-    const double* __restrict__ src = slab + (long long)max(s, 0) * (P * K) + lane_src;
 #pragma unroll
-    for (int t = 0; t < K; t++) dstv[t] = src[t * P];
+    for (int t = 0; t < K; t++)
+      dstv[t] = slab_load(srsrc, off_src, static_cast<unsigned>((max(s, 0) * (P * K) + t * P) * 8));
   };
 #pragma unroll
   for (int e = 0; e < PD; e++) fetch(half - 1 - e, ring[e]);
