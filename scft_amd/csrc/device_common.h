@@ -24,6 +24,25 @@ __device__ __forceinline__ double row_shr(double v) { return dpp_d<0x110 + S>(v)
 template <int S>
 __device__ __forceinline__ double row_shl(double v) { return dpp_d<0x100 + S>(v); }
 
+// GFX9 broadcast and wave-shift DPP moves on a double.  Rows outside
+// ROWMASK are not written and read 0 (the `old` operand); bound_ctrl makes a
+// lane without a source read 0.
+template <int CTRL, int ROWMASK, bool BOUND>
+__device__ __forceinline__ double dpp_rows_d(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b), CTRL, ROWMASK, 0xF, BOUND);
+  const int hi =
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, ROWMASK, 0xF, BOUND);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+// rows 1 and 3 read lane 15 of the row before (rows 0, 2: 0)
+__device__ __forceinline__ double row_bcast15(double v) { return dpp_rows_d<0x142, 0xA, false>(v); }
+// rows 2 and 3 read lane 31 (rows 0, 1: 0)
+__device__ __forceinline__ double row_bcast31(double v) { return dpp_rows_d<0x143, 0xC, false>(v); }
+// lane p reads lane p - 1 across the whole wave (lane 0: 0)
+__device__ __forceinline__ double wave_shr1(double v) { return dpp_rows_d<0x138, 0xF, true>(v); }
+
 // value of `v` in lane `l`, as a wave-uniform scalar
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);

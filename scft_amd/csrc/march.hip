@@ -270,6 +270,12 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
                         : (row == 1) ? Grow2 * Grow3 : Grow1 * Grow2 * Grow3;
   // wave prefix over lanes 0..lane-1 and suffix over lane+1..63 (wave folds)
   const double Apre_m1 = ((rl > 0) ? Ashift : 1.0) * Abefore;
+  // multipliers of the forward scan's two broadcast levels: rows 1 and 3 take
+  // the prefix ending at lane 15 / 47 times their own row product up to the
+  // lane; then rows 2 and 3 take the prefix ending at lane 31 times the
+  // product from lane 32 up to the lane
+  const double mf5 = Af;
+  const double mf6 = (row == 3) ? Arow2 * Af : Af;
   const double Gsuf_p1 = ((rl < 15) ? Gshift : 1.0) * Gafter;
   double gl = __shfl_up(ga[K - 1], 1);  // gamma of the row left of me
   // Fold multipliers.  The forward carry into this wave from the waves before
@@ -331,15 +337,13 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     B = fma(mf[1], row_shr<2>(B), B);
     B = fma(mf[2], row_shr<4>(B), B);
     B = fma(mf[3], row_shr<8>(B), B);
-    // carries into rows 1..3: the prefix up to the last lane of the row before
-    const double c1 = readlane_d(B, 15);
-    const double c2 = fma(Arow1, c1, readlane_d(B, 31));
-    const double c3 = fma(Arow2, c2, readlane_d(B, 47));
-    const double cm = (row == 0) ? 0.0 : (row == 1) ? c1 : (row == 2) ? c2 : c3;
-    const double B_sh = row_shr<1>(B);
-    double zin = (rl == 0) ? cm : fma(Ashift, cm, B_sh);
+    // rows to the whole wave: broadcast levels instead of readlane carries
+    B = fma(mf5, row_bcast15(B), B);
+    B = fma(mf6, row_bcast31(B), B);
+    // B is now the wave-inclusive prefix; this lane's input is lane - 1's
+    double zin = wave_shr1(B);
     if constexpr (W > 1) {
-      sh.Bw[wave] = fma(Arow3, c3, readlane_d(B, 63));
+      sh.Bw[wave] = readlane_d(B, 63);
       wg_barrier();
       double Z0 = 0.0, Z1 = 0.0;
 #pragma unroll
