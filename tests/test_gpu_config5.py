@@ -1,0 +1,82 @@
+"""Configuration 5, the production mix: 16384 instances on graded meshes
+h(x) = h0 (1 + x/ell), h0 log-uniform in [1e-4, 1e-2], ell in [0.05, 0.5],
+64-4096 nodes, N in {512, ..., 8192} (scft_amd.workloads.config5).
+
+* One evaluation sweep of all 16384 instances on the device, audited on 64
+  random instances against the CPU oracle (|phi_dev - phi_cpu| <= 1e-10; the
+  oracle's own FP64 march agrees with its extended-precision march to ~1e-12
+  on the stiffest of these shapes, dt/h0^2 ~ 2e5).
+* A batched solve of a 1024-instance sample with the preconditioned log
+  mixer.  The stiffest instances (dt/h0^2 >~ 1e4, where the converged field
+  at the first node reaches 1e6-1e8) do not all converge with this mixer: in
+  this sample 920 of 1024 do (every failure has dt/h0^2 >= 1.4e4).  The test
+  prints the fraction, requires at least 85%, and checks converged fields
+  against the oracle's evaluator.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _scft():
+    import scft_amd
+    return scft_amd
+
+
+def _cfg():
+    from scft_amd import workloads as WL
+    cfg = WL.config5(n_inst=16384)
+    for w in cfg["w0"]:
+        w[0] = 0.0
+    return cfg
+
+
+def test_config5_full_sweep_audited():
+    S = _scft()
+    cfg = _cfg()
+    print(f"config5: {len(cfg['xs'])} instances", flush=True)
+    b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN)
+    out = b.eval([w[1:] for w in cfg["w0"]])
+    b.close()
+    assert (out["status"] == 0).all(), np.unique(out["status"])
+    rng = np.random.default_rng(64)
+    idx = rng.choice(len(cfg["xs"]), 64, replace=False)
+    ref = O.batch_eval([len(cfg["xs"][k]) for k in idx], [cfg["Ns"][k] for k in idx],
+                       [cfg["xs"][k] for k in idx], [cfg["pts"][k] for k in idx],
+                       [cfg["w0"][k] for k in idx], bc=O.BC_DN, threads=16)
+    assert ref["rc"] == 0
+    for j, k in enumerate(idx):
+        np.testing.assert_allclose(out["phi"][k], ref["phi"][j][1:], rtol=0, atol=1e-10,
+                                   err_msg=f"instance {k} M={len(cfg['xs'][k])} N={cfg['Ns'][k]}")
+        assert abs(out["rnorm"][k] - ref["rnorm"][j]) <= 1e-10
+
+
+def test_config5_sample_solve():
+    S = _scft()
+    cfg = _cfg()
+    rng = np.random.default_rng(5)
+    idx = rng.choice(len(cfg["xs"]), 1024, replace=False)
+    xs = [cfg["xs"][k] for k in idx]
+    Ns = [cfg["Ns"][k] for k in idx]
+    pts = [cfg["pts"][k] for k in idx]
+    b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
+    opts = S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=200, step_cap=0.0,
+                           transform=S.MIX_LOG, floor=1e-8, precond=0.5, backtrack=1e30,
+                           check_every=4)
+    out = b.solve([cfg["w0"][k][1:] for k in idx], opts)
+    b.close()
+    conv = (out["status"] & S.ST_CONVERGED) != 0
+    h0 = np.array([x[1] - x[0] for x in xs])
+    stiff = 1.0 / np.array(Ns) / h0**2
+    print(f"config5 sample: {conv.sum()}/{len(idx)} converged, mean iters "
+          f"{out['iters'][conv].mean():.1f}, max {out['iters'][conv].max()}; unconverged "
+          f"dt/h0^2 min {stiff[~conv].min() if (~conv).any() else 0:.2e}", flush=True)
+    assert conv.mean() >= 0.85
+    # the converged fields solve the oracle's evaluator to the tolerance
+    for j in np.flatnonzero(conv)[:: max(1, conv.sum() // 16)][:16]:
+        wf = np.concatenate([[0.0], out["w"][j]])
+        r = O.evaluate(xs[j], wf, pts[j], Ns[j], bc=O.BC_DN)["rnorm"]
+        assert r <= 1.05e-6, (j, r)
