@@ -43,6 +43,19 @@ __device__ __forceinline__ double row_bcast31(double v) { return dpp_rows_d<0x14
 // lane p reads lane p - 1 across the whole wave (lane 0: 0)
 __device__ __forceinline__ double wave_shr1(double v) { return dpp_rows_d<0x138, 0xF, true>(v); }
 
+// lane p reads lane p + 1 across the whole wave (lane 63: 0)
+__device__ __forceinline__ double wave_shl1(double v) { return dpp_rows_d<0x130, 0xF, true>(v); }
+
+// value of lane (byte_addr / 4) mod 64 through the LDS crossbar (ds_bpermute:
+// no VALU issue, no LDS storage)
+__device__ __forceinline__ double bpermute_d(int byte_addr, double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(byte_addr, static_cast<int>(b));
+  const int hi = __builtin_amdgcn_ds_bpermute(byte_addr, static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) |
+                              static_cast<unsigned int>(lo));
+}
+
 // value of `v` in lane `l`, as a wave-uniform scalar
 __device__ __forceinline__ double readlane_d(double v, int l) {
   const long long b = __double_as_longlong(v);

@@ -276,6 +276,15 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   // product from lane 32 up to the lane
   const double mf5 = Af;
   const double mf6 = (row == 3) ? Arow2 * Af : Af;
+  // the backward scan's two cross-row levels, mirror images of those (DPP has
+  // no downward broadcast: the values come through ds_bpermute): rows 0 and 2
+  // take the suffix starting at lane 16 / 48 times their own row product
+  // from the lane on; then rows 0 and 1 take the suffix starting at lane 32
+  // times the product from the lane to lane 31.  0 for rows that do not take.
+  const double mbA = (row == 0 || row == 2) ? Gb : 0.0;
+  const double mbB = (row == 0) ? Gb * Grow1 : (row == 1) ? Gb : 0.0;
+  const int srcA = ((row + 1) & 3) * 64;  // byte address of lane 16 (row + 1)
+  const int srcB = 32 * 4;
   const double Gsuf_p1 = ((rl < 15) ? Gshift : 1.0) * Gafter;
   double gl = __shfl_up(ga[K - 1], 1);  // gamma of the row left of me
   // Fold multipliers.  The forward carry into this wave from the waves before
@@ -370,14 +379,12 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     Cc = fma(mb[2], row_shl<4>(Cc), Cc);
     Cc = fma(mb[3], row_shl<8>(Cc), Cc);
     // carries into rows 0..2: the suffix from the first lane of the row after
-    const double e2 = readlane_d(Cc, 48);
-    const double e1 = fma(Grow2, e2, readlane_d(Cc, 32));
-    const double e0 = fma(Grow1, e1, readlane_d(Cc, 16));
-    const double em = (row == 3) ? 0.0 : (row == 2) ? e2 : (row == 1) ? e1 : e0;
-    const double C_sh = row_shl<1>(Cc);
-    double xin = (rl == 15) ? em : fma(Gshift, em, C_sh);
+    Cc = fma(mbA, bpermute_d(srcA, Cc), Cc);
+    Cc = fma(mbB, bpermute_d(srcB, Cc), Cc);
+    // Cc is now the wave-inclusive suffix; this lane's input is lane + 1's
+    double xin = wave_shl1(Cc);
     if constexpr (W > 1) {
-      sh.Cw[wave] = fma(Grow0, e0, readlane_d(Cc, 0));
+      sh.Cw[wave] = readlane_d(Cc, 0);
       wg_barrier();
       double X0 = 0.0, X1 = 0.0;
 #pragma unroll
