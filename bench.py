@@ -52,12 +52,14 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-# The mixer of both sides: history 10, log(phi/phi_t) mixing residual
-# preconditioned by I + 0.5 M_L^{-1} K, beta 1, no step cap, a NaN evaluation
-# halves the step (backtrack factor 1e30: NaN only).
-SOLVER = dict(history=10, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8, precond=0.5,
+# The mixer of both sides: history 16 (the fine meshes' wall layer has many
+# slowly converging modes: history 10 needs up to 32 evaluations on this batch,
+# 16 needs 25), log(phi/phi_t) mixing residual preconditioned by
+# I + 0.5 M_L^{-1} K, beta 1, no step cap, a NaN evaluation halves the step
+# (backtrack factor 1e30: NaN only).
+SOLVER = dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8, precond=0.5,
               backtrack=1e30, max_halvings=30)
-SOLVER_TEXT = ("device Anderson, history 10, log mixing preconditioned by I + 0.5 M^-1 K, "
+SOLVER_TEXT = ("device Anderson, history 16, log mixing preconditioned by I + 0.5 M^-1 K, "
                "beta 1.0, no step cap, NaN backtracking")
 
 

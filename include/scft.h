@@ -156,7 +156,7 @@ int scft_debug_slabs(scft_batch *b, int inst, double *out);
  *   r mixed as is / relative / log(phi/phi_t), the step capped in inf-norm,
  *   history cleared on growth past restart_factor * best or singular U. */
 typedef struct scft_anderson_opts {
-  int history;            /* 10 */
+  int history;            /* previous iterates in the Gram system, 1..16 (10) */
   double beta;            /* 0.1 */
   double tol;             /* 1e-6 */
   int max_iter;           /* evaluation limit per instance */

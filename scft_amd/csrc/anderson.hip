@@ -28,7 +28,7 @@
 namespace scft {
 
 constexpr int AM_THREADS = 256;
-constexpr int AM_MAXH = 12;  // previous iterates supported (history <= 12)
+constexpr int AM_MAXH = 16;  // previous iterates supported (history <= 16)
 
 __device__ __forceinline__ double mix_residual(double r, double pt, int transform,
                                                double floor) {
@@ -43,7 +43,7 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   const int tid = threadIdx.x;
   __shared__ double U[AM_MAXH * AM_MAXH];
   __shared__ double bvec[AM_MAXH];
-  __shared__ double red[AM_THREADS / 64][AM_MAXH * (AM_MAXH + 1) / 2 + AM_MAXH];
+  __shared__ double red[AM_THREADS / 64][16 * 16 + 16];
   __shared__ double red1[AM_THREADS / 64];
   __shared__ int s_m, s_mode;
   __shared__ double s_scale;
@@ -163,55 +163,53 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   __syncthreads();
 
   // ----------------------------------------------------- Gram system
+  // U_ij = sum_u dv_i dv_j and b_i = sum_u dv_i r_k with dv_i = r_k - r_{k-1-i}
+  // on the FP64 matrix cores: v_mfma_f64_16x16x4f64 on a 16 x 4 block of dv
+  // (unknowns u0 .. u0+3), lane l holding dv_{l%16}(u0 + l/16), which is both
+  // its A element (row l%16, k = l/16) and its B element (k = l/16, column
+  // l%16); a second product against r_k in column 0 gives b.  One pass over
+  // the history, 8 accumulator doubles per lane, m <= 16.
   if (m > 0) {
-    constexpr int NP = AM_MAXH * (AM_MAXH + 1) / 2 + AM_MAXH;
-    double part[NP];
-#pragma unroll
-    for (int e = 0; e < NP; e++) part[e] = 0.0;
-    int slot[AM_MAXH];
-#pragma unroll
-    for (int i = 0; i < AM_MAXH; i++) slot[i] = ((cur - (i + 1)) % S + S) % S;
-    for (int u = tid; u < n; u += AM_THREADS) {
-      const double rk = hr[(long long)cur * n + u];
-      double dv[AM_MAXH];
-#pragma unroll
-      for (int i = 0; i < AM_MAXH; i++)
-        dv[i] = (i < m) ? rk - hr[(long long)slot[i] * n + u] : 0.0;
-      int e = 0;
-#pragma unroll
-      for (int i = 0; i < AM_MAXH; i++) {
-#pragma unroll
-        for (int j = 0; j <= i; j++) part[e++] += dv[i] * dv[j];
-      }
-#pragma unroll
-      for (int i = 0; i < AM_MAXH; i++) part[e++] += dv[i] * rk;
-    }
+    typedef double d4 __attribute__((ext_vector_type(4)));
     const int lane = tid & 63, wave = tid >> 6;
+    const int vi = lane & 15, ku = lane >> 4;
+    const int slot = ((cur - (vi + 1)) % S + S) % S;
+    const double* __restrict__ hk = hr + (long long)cur * n;
+    const double* __restrict__ hi = hr + (long long)slot * n;
+    const bool use = vi < m;
+    // each wave takes a contiguous range of unknowns, four at a time
+    constexpr int NW = AM_THREADS / 64;
+    const int per = ((n + 4 * NW - 1) / (4 * NW)) * 4;
+    const int ub = wave * per, ue = min(n, ub + per);
+    d4 accG = {0.0, 0.0, 0.0, 0.0}, accB = {0.0, 0.0, 0.0, 0.0};
+    for (int u0 = ub; u0 < ue; u0 += 4) {
+      const int u = u0 + ku;
+      const bool ok = u < ue;
+      const int uc = ok ? u : ub;  // in-range index for the masked lanes
+      const double rk = ok ? hk[uc] : 0.0;
+      const double hv = hi[uc];
+      const double dv = (ok && use) ? rk - hv : 0.0;
+      accG = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, dv, accG, 0, 0, 0);
+      accB = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, vi == 0 ? rk : 0.0, accB, 0, 0, 0);
+    }
+    // result layout: lane l, element e holds row l/16 + 4 e, column l%16
 #pragma unroll
-    for (int e = 0; e < NP; e++) {
-      double v = part[e];
-#pragma unroll
-      for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-      if (lane == 0) red[wave][e] = v;
+    for (int e = 0; e < 4; e++) {
+      red[wave][(ku + 4 * e) * 16 + vi] = accG[e];
+      if (vi == 0) red[wave][256 + ku + 4 * e] = accB[e];
+    }
+    __syncthreads();
+    for (int e = tid; e < m * m + m; e += AM_THREADS) {
+      const int idx = (e < m * m) ? (e / m) * 16 + (e % m) : 256 + (e - m * m);
+      double t = 0.0;
+      for (int v = 0; v < NW; v++) t += red[v][idx];
+      if (e < m * m)
+        U[e] = t;
+      else
+        bvec[e - m * m] = t;
     }
     __syncthreads();
     if (tid == 0) {
-      double tot[NP];
-      for (int e = 0; e < NP; e++) {
-        double s = 0.0;
-        for (int v = 0; v < AM_THREADS / 64; v++) s += red[v][e];
-        tot[e] = s;
-      }
-      int e = 0;
-      for (int i = 0; i < AM_MAXH; i++)
-        for (int j = 0; j <= i; j++) {
-          if (i < m && j < m) U[i * m + j] = U[j * m + i] = tot[e];
-          e++;
-        }
-      for (int i = 0; i < AM_MAXH; i++) {
-        if (i < m) bvec[i] = tot[e];
-        e++;
-      }
       double tr = 0.0;
       for (int i = 0; i < m; i++) tr += U[i * m + i];
       for (int i = 0; i < m; i++) U[i * m + i] += a.reg * tr;

@@ -97,6 +97,6 @@ int irk4_max_grid(int K, int W, int half_max);  // 0: the workgroup does not fit
 bool irk4_variant_exists(int K, int W);
 hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_reset(int* counters, int n, hipStream_t s);
-constexpr int MIX_MAX_HISTORY = 12;
+constexpr int MIX_MAX_HISTORY = 16;
 
 }  // namespace scft

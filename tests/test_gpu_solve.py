@@ -55,7 +55,10 @@ def test_device_anderson_trajectory_matches_oracle(transform, beta, cap, precond
 def test_device_anderson_converges_config1():
     """Config 1 (m = 32, N = 2048, w0 = 0), history 10, mixing 0.1, cap 0.5:
     converges on the device and the converged field evaluates (on the CPU
-    oracle) to the same residual."""
+    oracle) to the same residual.  The evaluation count of this plain mixer is
+    chaotic in the rounding (perturbing w0 by 1e-14 moves the oracle's count
+    between ~310 and ~580), so the device's count is checked against the band
+    the oracle itself spans over such perturbations."""
     S = _S()
     cfg = WL.config1()
     x, N, pt, w0 = cfg["xs"][0], cfg["Ns"][0], cfg["pts"][0], cfg["w0"][0]
@@ -67,8 +70,16 @@ def test_device_anderson_converges_config1():
     assert ref["rnorm"] <= 1e-6
     cpu = O.solve_anderson(x, pt, N, w0, O.anderson_opts(max_iter=3000), bc=O.BC_DN)
     assert cpu["rc"] == 0
-    print(f"config1: device {out['iters'][0]} evaluations, oracle {cpu['iterations']}")
-    assert abs(int(out["iters"][0]) - cpu["iterations"]) <= 0.25 * cpu["iterations"] + 20
+    rng = np.random.default_rng(0)
+    band = [cpu["iterations"]]
+    for _ in range(5):
+        wp = w0.copy()
+        wp[1:] += rng.normal(0.0, 1e-14, len(wp) - 1)
+        band.append(O.solve_anderson(x, pt, N, wp, O.anderson_opts(max_iter=3000),
+                                     bc=O.BC_DN)["iterations"])
+    print(f"config1: device {out['iters'][0]} evaluations, oracle {cpu['iterations']}, "
+          f"oracle under 1e-14 perturbations {sorted(band)}")
+    assert 0.75 * min(band) <= int(out["iters"][0]) <= 1.25 * max(band)
     # phi of the two converged fields agree to the tolerance they were solved to
     e2 = O.evaluate(x, cpu["w"], pt, N, bc=O.BC_DN)
     assert np.abs(ref["phi"] - e2["phi"]).max() <= 2e-6
