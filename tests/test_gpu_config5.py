@@ -9,7 +9,7 @@ h(x) = h0 (1 + x/ell), h0 log-uniform in [1e-4, 1e-2], ell in [0.05, 0.5],
 * A batched solve of a 1024-instance sample with the preconditioned log
   mixer.  The stiffest instances (dt/h0^2 >~ 1e4, where the converged field
   at the first node reaches 1e6-1e8) do not all converge with this mixer: in
-  this sample 920 of 1024 do (every failure has dt/h0^2 >= 1.4e4).  The test
+  this sample about 90% do (every failure has dt/h0^2 >= 1.4e4).  The test
   prints the fraction, requires at least 85%, and checks converged fields
   against the oracle's evaluator.
 """
@@ -63,7 +63,7 @@ def test_config5_sample_solve():
     Ns = [cfg["Ns"][k] for k in idx]
     pts = [cfg["pts"][k] for k in idx]
     b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
-    opts = S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=200, step_cap=0.0,
+    opts = S.anderson_opts(history=16, beta=1.0, tol=1e-6, max_iter=200, step_cap=0.0,
                            transform=S.MIX_LOG, floor=1e-8, precond=0.5, backtrack=1e30,
                            check_every=4)
     out = b.solve([cfg["w0"][k][1:] for k in idx], opts)
