@@ -178,3 +178,34 @@ def test_dropin_evaluator_matches_oracle():
     S.lib.scft_adm_chen_f(n, np.ascontiguousarray(w_nr[1:]), o2)
     np.testing.assert_allclose(o2, -ref["r"][1:-1], rtol=0, atol=PHI_ATOL)
     S.finalize()
+
+
+def test_consistent_mass_with_wall_values():
+    """Consistent w-mass (scft.cc:641-651) with nonzero field values at the
+    walls, as the deal.II path extrapolates them (scft.cc:454-480), installed
+    through scft_batch_set_walls; both boundary conditions."""
+    S = _scft()
+    rng = np.random.default_rng(21)
+    for bc in (O.BC_DD, O.BC_DN):
+        m, N, L = 300, 512, 3.5
+        x = O.mesh_uniform(m + 1, L)
+        pt = O.target_position(x, O.TAU, bc == O.BC_DD)
+        w = 0.5 * np.cos(2.0 * x) + rng.normal(0.0, 0.05, m + 1)
+        w[0], w[-1] = 1.3, -0.7
+        ref = O.evaluate(x, w, pt, N, bc=bc, cmat=O.C_CONSISTENT, want_q=True)
+        b = S.Batch([x], [N], [unknowns(pt, bc)], bc=bc, cmat=S.C_CONSISTENT)
+        b.set_walls([w[0]], [w[-1]])
+        out = b.eval([unknowns(w, bc)])
+        slabs = b.debug_slabs(0)
+        qc = ref["q"][1:N // 2]
+        qc = qc[:, 1:-1] if bc == O.BC_DD else qc[:, 1:]
+        err = np.abs(slabs - qc).max(axis=1) / np.abs(qc).max(axis=1)
+        assert err.max() <= STEP_RTOL, (bc, err.max())
+        np.testing.assert_allclose(out["phi"][0], unknowns(ref["phi"], bc), rtol=0, atol=PHI_ATOL)
+        # the walls matter: zero walls change the density by ~1e-7 here (the
+        # wall element's w-mass multiplies a propagator that vanishes at the
+        # wall), far above the 1e-10 parity bar
+        b.set_walls([0.0], [0.0])
+        out0 = b.eval([unknowns(w, bc)])
+        assert np.abs(out0["phi"][0] - out["phi"][0]).max() > 1e-8
+        b.close()
