@@ -133,7 +133,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   wg_barrier();
 
   // ---------------------------------------------------------------- assembly
-  double av[K], bv[K], cv[K], ra[K], rc[K];
+  double av[K], bv[K], cv[K], ra[K], rc[K], rd[K];
 #pragma unroll
   for (int t = 0; t < K; t++) {
     const Row rw = assemble_row(x, w, n, M, u0 + t, dt, cmat, d.wall0, d.wallL);
@@ -214,6 +214,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
       ga[t] = cv[t] * inv;
       ra[t] *= inv;
       rc[t] *= inv;
+      rd[t] = 2.0 * (ra[t] + rc[t]);
       cp = cv[t];
       up = uu;
     }
@@ -336,7 +337,8 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     for (int t = 0; t < K; t++) {
       const double qm = (t == 0) ? ql : q[t - 1];
       const double qp = (t == K - 1) ? qr : q[t + 1];
-      rp[t] = ra[t] * (qm + 2.0 * q[t]) + rc[t] * (2.0 * q[t] + qp);
+      // (A q)_i / u_i = ra q_{i-1} + 2 (ra + rc) q_i + rc q_{i+1}
+      rp[t] = fma(rc[t], qp, fma(rd[t], q[t], ra[t] * qm));
     }
     // forward recurrence z_i = r'_i - alpha_i z_{i-1}
     double B = 0.0;
