@@ -64,6 +64,13 @@ template <int S>
 __device__ __forceinline__ cd c_row_shl(cd v) { return {row_shl<S>(v.re), row_shl<S>(v.im)}; }
 __device__ __forceinline__ cd c_readlane(cd v, int l) { return {readlane_d(v.re, l), readlane_d(v.im, l)}; }
 __device__ __forceinline__ cd c_shfl_up(cd v, int s) { return {__shfl_up(v.re, s), __shfl_up(v.im, s)}; }
+__device__ __forceinline__ cd c_row_bcast15(cd v) { return {row_bcast15(v.re), row_bcast15(v.im)}; }
+__device__ __forceinline__ cd c_row_bcast31(cd v) { return {row_bcast31(v.re), row_bcast31(v.im)}; }
+__device__ __forceinline__ cd c_wave_shr1(cd v) { return {wave_shr1(v.re), wave_shr1(v.im)}; }
+__device__ __forceinline__ cd c_wave_shl1(cd v) { return {wave_shl1(v.re), wave_shl1(v.im)}; }
+__device__ __forceinline__ cd c_bpermute(int byte_addr, cd v) {
+  return {bpermute_d(byte_addr, v.re), bpermute_d(byte_addr, v.im)};
+}
 
 // ------------------------------------------------ complex double-double
 struct cdd {
@@ -345,6 +352,14 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
                            c_sel(row == 1, G23, c_mul(Grow1, G23))));
   const cd Apre_m1 = c_mul(c_sel(rl > 0, Ashift, c1v), Abefore);
   const cd Gsuf_p1 = c_mul(c_sel(rl < 15, Gshift, c1v), Gafter);
+  // multipliers of the cross-row levels (march.hip): forward through the
+  // broadcast moves, backward through ds_bpermute
+  const cd mf5 = Af;
+  const cd mf6 = c_sel(row == 3, c_mul(Arow2, Af), Af);
+  const cd mbA = c_sel(row == 0 || row == 2, Gb, c0);
+  const cd mbB = c_sel(row == 0, c_mul(Gb, Grow1), c_sel(row == 1, Gb, c0));
+  const int srcA = ((row + 1) & 3) * 64;
+  const int srcB = 32 * 4;
   cd gl = c_shfl_up(ga[K - 1], 1);
   if constexpr (W > 1) {
     sh.Aw[wave] = c_mul(A01, c_mul(Arow2, Arow3));
@@ -399,14 +414,11 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
     B = c_fma(mf[1], c_row_shr<2>(B), B);
     B = c_fma(mf[2], c_row_shr<4>(B), B);
     B = c_fma(mf[3], c_row_shr<8>(B), B);
-    const cd k1 = c_readlane(B, 15);
-    const cd k2 = c_fma(Arow1, k1, c_readlane(B, 31));
-    const cd k3 = c_fma(Arow2, k2, c_readlane(B, 47));
-    const cd cm = c_sel(row == 0, c0, c_sel(row == 1, k1, c_sel(row == 2, k2, k3)));
-    const cd B_sh = c_row_shr<1>(B);
-    cd zin = c_sel(rl == 0, cm, c_fma(Ashift, cm, B_sh));
+    B = c_fma(mf5, c_row_bcast15(B), B);
+    B = c_fma(mf6, c_row_bcast31(B), B);
+    cd zin = c_wave_shr1(B);
     if constexpr (W > 1) {
-      sh.Bw[wave] = c_fma(Arow3, k3, c_readlane(B, 63));
+      sh.Bw[wave] = c_readlane(B, 63);
       wg_barrier();
       cd Z = c0;
 #pragma unroll
@@ -429,14 +441,11 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
     Cc = c_fma(mb[1], c_row_shl<2>(Cc), Cc);
     Cc = c_fma(mb[2], c_row_shl<4>(Cc), Cc);
     Cc = c_fma(mb[3], c_row_shl<8>(Cc), Cc);
-    const cd e2 = c_readlane(Cc, 48);
-    const cd e1 = c_fma(Grow2, e2, c_readlane(Cc, 32));
-    const cd e0 = c_fma(Grow1, e1, c_readlane(Cc, 16));
-    const cd em = c_sel(row == 3, c0, c_sel(row == 2, e2, c_sel(row == 1, e1, e0)));
-    const cd C_sh = c_row_shl<1>(Cc);
-    cd xin = c_sel(rl == 15, em, c_fma(Gshift, em, C_sh));
+    Cc = c_fma(mbA, c_bpermute(srcA, Cc), Cc);
+    Cc = c_fma(mbB, c_bpermute(srcB, Cc), Cc);
+    cd xin = c_wave_shl1(Cc);
     if constexpr (W > 1) {
-      sh.Cw[wave] = c_fma(Grow0, e0, c_readlane(Cc, 0));
+      sh.Cw[wave] = c_readlane(Cc, 0);
       wg_barrier();
       cd X = c0;
 #pragma unroll
