@@ -23,8 +23,9 @@
 //           rho = -dt omega / u;
 //   step    r'_i = rho_i (D' q)_i, z_i = r'_i - alpha_i z_{i-1},
 //           x_i = z_i - gamma_i x_{i+1}, q_i += Re x_i; each recurrence as a
-//           lane-local pass, a 16-lane DPP row scan, readlane row carries and
-//           a fold over waves through LDS, with every multiplier fixed per
+//           lane-local pass, a 16-lane DPP row scan, two cross-row levels
+//           (broadcast DPP forward, LDS-crossbar permutes backward) and a
+//           fold over waves through LDS, with every multiplier fixed per
 //           instance;
 //   density and residual exactly as march.hip: slabs 1 .. N/2-1 streamed to
 //           HBM and paired with steps N/2+1 .. N through a register ring,

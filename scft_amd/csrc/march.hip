@@ -20,14 +20,17 @@
 //   3. march:    N implicit-Euler steps D q^{j+1} = A q^j (1D_FEM.c:208-228,
 //                step count of DEALII_SCFT/drivescft.cc:130).  Each first-order
 //                recurrence is split into a lane-local pass, a scan of the
-//                lanes' affine maps (6 shuffle levels whose multipliers are
-//                fixed per instance, so only one FMA per level per step), a
-//                fold over waves through LDS, and a lane-local re-run.
+//                lanes' affine maps over the wave (four DPP row levels, then
+//                two cross-row levels: broadcast DPP moves forward, LDS-
+//                crossbar permutes backward; every multiplier is fixed per
+//                instance, so a level costs one FMA per step), a fold over
+//                waves through LDS in one round trip, and a lane-local re-run.
 //   4. density:  phi_i = sum_j c_j q_j(i) q_{N-j}(i) (drivescft.cc:184-193 with
 //                romint.c's weights, symmetric in j).  Slabs q_1..q_{N/2-1}
-//                stream to HBM (coalesced: element t of lane p at t*P + p);
-//                from step N/2 on each new slab is paired with the stored
-//                slab N-j, prefetched one step ahead.
+//                stream to HBM (coalesced: element t of lane p at t*P + p,
+//                raw-buffer stores and loads with non-temporal hints); from
+//                step N/2 on each new slab is paired with the stored slab
+//                N-j, loaded a few steps ahead through a register ring.
 //   5. residual: r = phi - phi_t (1D_FEM.c:274-277), ||r||_inf per instance.
 //
 // No MFMA: nothing here is a dense contraction.
@@ -225,10 +228,10 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   // x_first = Gb x_next + C, with Af = prod(-alpha), Gb = prod(-gamma) fixed
   // for the instance.  The per-step scans combine these maps over the wave:
   // a Hillis-Steele scan inside each 16-lane row (DPP row_shr / row_shl by
-  // 1, 2, 4, 8), then the three row-to-row carries from readlane values and
-  // the fixed row products, then (several waves) a fold over wave totals in
-  // LDS.  All multipliers are precomputed here; a step costs one FMA per
-  // level plus the carries.
+  // 1, 2, 4, 8), then two row-to-row levels (forward: DPP row_bcast:15 and
+  // row_bcast:31; backward: ds_bpermute from the first lane of the next
+  // rows), then (several waves) a fold over wave totals in LDS.  All
+  // multipliers are precomputed here; a step costs one FMA per level.
   const int rl = lane & 15;   // lane within its row
   const int row = lane >> 4;  // row within the wave
   double Af = 1.0, Gb = 1.0;
