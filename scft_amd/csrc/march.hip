@@ -136,7 +136,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   wg_barrier();
 
   // ---------------------------------------------------------------- assembly
-  double av[K], bv[K], cv[K], ra[K], rc[K], rd[K];
+  double av[K], bv[K], cv[K], ra[K], rc[K];
 #pragma unroll
   for (int t = 0; t < K; t++) {
     const Row rw = assemble_row(x, w, n, M, u0 + t, dt, cmat, d.wall0, d.wallL);
@@ -217,7 +217,6 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
       ga[t] = cv[t] * inv;
       ra[t] *= inv;
       rc[t] *= inv;
-      rd[t] = 2.0 * (ra[t] + rc[t]);
       cp = cv[t];
       up = uu;
     }
@@ -239,6 +238,23 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   for (int t = 0; t < K; t++) {
     Af *= -al[t];
     Gb *= -ga[t];
+  }
+  // Within the chunk, z_t = zloc_t + Pf[t] zin with zloc the local pass from
+  // zin = 0 and Pf[t] = prod_{s <= t} (-alpha_s); so the lane's backward offset
+  // (the local backward pass over z) is C0 + Sz zin, C0 the local backward
+  // pass over zloc and Sz = sum_t prod_{s < t} (-gamma_s) Pf[t].  C0 needs no
+  // zin: it runs while the forward scan is in flight, and the final forward
+  // pass leaves the critical path.
+  double Pf[K], Sz = 0.0;
+  {
+    double p = 1.0, g = 1.0;
+#pragma unroll
+    for (int t = 0; t < K; t++) {
+      p *= -al[t];
+      Pf[t] = p;
+      Sz = fma(g, p, Sz);
+      g *= -ga[t];
+    }
   }
   double mf[4], mb[4];
   {
@@ -340,13 +356,19 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     for (int t = 0; t < K; t++) {
       const double qm = (t == 0) ? ql : q[t - 1];
       const double qp = (t == K - 1) ? qr : q[t + 1];
-      // (A q)_i / u_i = ra q_{i-1} + 2 (ra + rc) q_i + rc q_{i+1}
-      rp[t] = fma(rc[t], qp, fma(rd[t], q[t], ra[t] * qm));
+      rp[t] = ra[t] * (qm + 2.0 * q[t]) + rc[t] * (2.0 * q[t] + qp);
     }
-    // forward recurrence z_i = r'_i - alpha_i z_{i-1}
+    // forward recurrence z_i = r'_i - alpha_i z_{i-1}: the local pass from 0
+    // (zloc kept in rp), and the local backward pass over it
     double B = 0.0;
 #pragma unroll
-    for (int t = 0; t < K; t++) B = rp[t] - al[t] * B;
+    for (int t = 0; t < K; t++) {
+      B = rp[t] - al[t] * B;
+      rp[t] = B;
+    }
+    double C0 = 0.0;
+#pragma unroll
+    for (int t = K - 1; t >= 0; t--) C0 = rp[t] - ga[t] * C0;
     B = fma(mf[0], row_shr<1>(B), B);
     B = fma(mf[1], row_shr<2>(B), B);
     B = fma(mf[2], row_shr<4>(B), B);
@@ -367,18 +389,8 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
       }
       zin = fma(Apre_m1, Z0 + Z1, zin);
     }
-    {
-      double z = zin;
-#pragma unroll
-      for (int t = 0; t < K; t++) {
-        z = rp[t] - al[t] * z;
-        rp[t] = z;
-      }
-    }
-    // backward recurrence x_i = z_i - gamma_i x_{i+1}
-    double Cc = 0.0;
-#pragma unroll
-    for (int t = K - 1; t >= 0; t--) Cc = rp[t] - ga[t] * Cc;
+    // backward recurrence x_i = z_i - gamma_i x_{i+1}: this lane's offset
+    double Cc = fma(Sz, zin, C0);
     Cc = fma(mb[0], row_shl<1>(Cc), Cc);
     Cc = fma(mb[1], row_shl<2>(Cc), Cc);
     Cc = fma(mb[2], row_shl<4>(Cc), Cc);
@@ -403,7 +415,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
       double xx = xin;
 #pragma unroll
       for (int t = K - 1; t >= 0; t--) {
-        xx = rp[t] - ga[t] * xx;
+        xx = fma(Pf[t], zin, rp[t]) - ga[t] * xx;  // z_t - gamma_t x_{t+1}
         q[t] = xx;
       }
     }
