@@ -21,6 +21,11 @@
  *               w == 0 with Neumann at both ends, dense LU on tiny M,
  *               mirror symmetry, half-domain DN == full-domain DD
  *   Mode M      solution_matlab_N=33 solves the MATLAB residual to ~2.3e-7
+ *   IRK4 march  (tests/test_oracle_irk4.py) the Gauss (2,2)-Pade factor on
+ *               sine modes, fourth-order convergence to expm, a dense solve
+ *               of the stage system, and the reference's N=33_for_read.txt:
+ *               a field deal.II's IRK solver converged to 1.42e-9 evaluates
+ *               here to 1.7e-9 (consistent C, N = 2048), 1.1e-4 with IE
  */
 #include "scft_oracle.h"
 
