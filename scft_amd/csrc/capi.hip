@@ -119,7 +119,21 @@ struct scft_batch {
   std::vector<MixState> ms0;
   int* h_active = nullptr;  // pinned
   int last_sweeps = 0;
+  // the sweep graph of the last resident solve, reused while its baked kernel
+  // arguments (bucket launches, mixer arguments) are unchanged
+  hipGraph_t sweep_graph = nullptr;
+  hipGraphExec_t sweep_exec = nullptr;
+  AndersonArgs sweep_args;
 };
+
+// Forget the cached sweep graph: the bucket launches or mixer buffers it
+// bakes in are about to change.
+static void drop_sweep_graph(scft_batch* b) {
+  if (b->sweep_exec) (void)hipGraphExecDestroy(b->sweep_exec);
+  if (b->sweep_graph) (void)hipGraphDestroy(b->sweep_graph);
+  b->sweep_exec = nullptr;
+  b->sweep_graph = nullptr;
+}
 
 // The march of the batch's stepper: implicit Euler (march.hip) or IRK4
 // (irk4.hip); both take the same arguments, slabs and buckets.
@@ -146,6 +160,7 @@ static double eval_bytes(int n, int N) {
 // concurrently): grid_b workgroups x N_max/2 slabs x 64 W K doubles.
 // Buckets are spread over the main and side streams, largest work first.
 static int size_scratch(scft_batch* b) {
+  drop_sweep_graph(b);
   long long need = 0;
   std::vector<std::pair<double, int>> order;
   for (size_t i = 0; i < b->buckets.size(); i++) {
@@ -411,6 +426,7 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
 extern "C" int scft_batch_destroy(scft_batch* b) {
   if (!b) return SCFT_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_sweep_graph(b);
   for (auto& bk : b->buckets) dfree(bk.d_items);
   dfree(b->d_desc);
   dfree(b->d_x);
@@ -681,6 +697,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
 
 static int ensure_mixer(scft_batch* b, int S) {
   if (b->hist_S == S && b->d_hw) return SCFT_OK;
+  drop_sweep_graph(b);
   dfree(b->d_hw);
   dfree(b->d_hr);
   dfree(b->d_ms);
@@ -726,6 +743,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   const auto t0 = std::chrono::steady_clock::now();
   const int nb = static_cast<int>(b->buckets.size());
   AndersonArgs a;
+  std::memset(&a, 0, sizeof a);  // padding included: the struct is compared bytewise
   a.desc = b->d_desc;
   a.w = b->d_w;
   a.r = b->d_r;
@@ -781,15 +799,23 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.fallback_thresh = 0.0;  // only the first evaluation may be rejected
 
   // remaining sweeps: [reset, march per bucket, mixer] captured once
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_sweep(b);
-  if (rc == SCFT_OK && launch_anderson(a, b->stream) != hipSuccess) rc = SCFT_E_HIP;
-  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
-  if (rc != SCFT_OK) return rc;
-  HIPCHK(ce);
-  HIPCHK(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  if (b->sweep_exec && std::memcmp(&a, &b->sweep_args, sizeof a) != 0) drop_sweep_graph(b);
+  if (!b->sweep_exec) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enqueue_sweep(b);
+    if (rc == SCFT_OK && launch_anderson(a, b->stream) != hipSuccess) rc = SCFT_E_HIP;
+    hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+    if (rc != SCFT_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    HIPCHK(ce);
+    b->sweep_graph = graph;
+    HIPCHK(hipGraphInstantiate(&b->sweep_exec, graph, nullptr, nullptr, 0));
+    b->sweep_args = a;
+  }
+  hipGraphExec_t exec = b->sweep_exec;
   if (!b->h_active) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int)));
   int sweeps = 1;
   const int check = std::max(1, o->check_every);
@@ -812,8 +838,6 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
       HIPCHK(hipGraphLaunch(exec, b->stream));
   }
   HIPCHK(hipStreamSynchronize(b->stream));
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
   const auto t1 = std::chrono::steady_clock::now();
   b->last_sweeps = sweeps;
   if (stats) {
