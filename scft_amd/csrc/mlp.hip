@@ -1,7 +1,7 @@
 // mlp.hip -- the warm-start MLP: dense FP32 layers (tanh on hidden layers,
 // linear output), mean-squared-error loss and Adam, trained and evaluated on
-// the device by the kernels below.  No reference counterpart (the survey's
-// row a12); parameters are packed as the FP64 restatement in
+// the device by the kernels below.  No reference counterpart: the reference
+// has no learned warm start.  Parameters are packed as the FP64 restatement in
 // oracle/oracle_mlp.c packs them: per layer W (dout x din, row-major), then
 // b (dout).
 //
