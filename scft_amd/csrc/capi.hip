@@ -96,10 +96,13 @@ struct scft_batch {
   std::string tune_file;
   std::vector<Bucket> buckets;
   hipStream_t stream = nullptr;  // main stream: copies, resets, the mixer
-  // buckets run concurrently on the main stream plus kSide side streams
-  static constexpr int kSide = 3;
-  hipStream_t side[kSide] = {nullptr, nullptr, nullptr};
-  hipEvent_t fork = nullptr, join[kSide] = {nullptr, nullptr, nullptr};
+  // buckets run concurrently on the main stream plus kSide side streams:
+  // one stream per bucket for the seven buckets of meshes up to 4096 nodes
+  // (a shared stream serialises its buckets: 13.2-13.7 ms per config-3 sweep
+  // on four streams against 12.5 ms on one stream each)
+  static constexpr int kSide = 7;
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
   // device state
   InstDesc* d_desc = nullptr;
   double *d_x = nullptr, *d_w = nullptr, *d_phit = nullptr, *d_pw = nullptr;
@@ -190,7 +193,7 @@ static int size_scratch(scft_batch* b) {
   }
   std::sort(order.begin(), order.end(), std::greater<std::pair<double, int>>());
   // greedy: next bucket onto the least loaded stream
-  double load[1 + scft_batch::kSide] = {0.0, 0.0, 0.0, 0.0};
+  double load[1 + scft_batch::kSide] = {};
   for (auto& o : order) {
     int s = 0;
     for (int t = 1; t <= scft_batch::kSide; t++)
@@ -255,16 +258,17 @@ static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_id
 // the main stream, then one persistent launch per non-empty bucket, the
 // buckets spread over the main and side streams (fork/join through events,
 // which stream capture turns into parallel graph branches).  The buckets are
-// launched smallest first: their short, latency-bound grids are then resident
-// beside the large buckets' persistent grids instead of queueing until those
-// drain (a config-3 sweep of 4096 instances: 13.2 ms against 14.0 ms largest
-// first).
+// launched smallest first, so that their short, latency-bound grids become
+// resident beside the large buckets' persistent grids.  That order matters
+// when buckets share a stream (a config-3 sweep of 4096 instances on four
+// streams: 13.2 ms against 14.0 ms largest first); with a stream per bucket
+// both orders measure 12.5 ms.
 static hipStream_t stream_of(scft_batch* b, int s) { return s == 0 ? b->stream : b->side[s - 1]; }
 
 static int enqueue_sweep(scft_batch* b) {
   const int nb = static_cast<int>(b->buckets.size());
   HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
-  bool used[1 + scft_batch::kSide] = {true, false, false, false};
+  bool used[1 + scft_batch::kSide] = {true};
   for (auto& bk : b->buckets)
     if (!bk.items.empty()) used[bk.stream] = true;
   HIPCHK(hipEventRecord(b->fork, b->stream));
