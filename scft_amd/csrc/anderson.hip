@@ -46,7 +46,9 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   __shared__ double red[AM_THREADS / 64][16 * 16 + 16];
   __shared__ double red1[AM_THREADS / 64];
   __shared__ int s_m, s_mode;
-  __shared__ double s_scale;
+  __shared__ double s_scale, s_tr;
+  __shared__ double s_pv[AM_THREADS / 64];  // per-wave pivot candidates
+  __shared__ int s_pi[AM_THREADS / 64];
 
   const unsigned st0 = __builtin_amdgcn_readfirstlane(a.status[inst]);
   if (st0 & (ST_CONVERGED | ST_MAXITER | ST_FAILED)) return;
@@ -213,57 +215,98 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
       double tr = 0.0;
       for (int i = 0; i < m; i++) tr += U[i * m + i];
       for (int i = 0; i < m; i++) U[i * m + i] += a.reg * tr;
-      // Gauss-Jordan with full pivoting (gaussj.c:7-60); zero pivot = singular
-      int ipiv[AM_MAXH], indxr[AM_MAXH], indxc[AM_MAXH];
-      bool sing = !(tr > 0.0);
-      for (int j = 0; j < m; j++) ipiv[j] = 0;
+      s_tr = tr;
+    }
+    __syncthreads();
+    // Gauss-Jordan with full pivoting (gaussj.c:7-60) on all threads: thread
+    // t owns element (t / 16, t % 16) of U and threads t < m the entry b_t.
+    // Each step takes the pivot the serial scan takes (the last largest |U_jk|
+    // in row-major order over rows and columns not yet pivoted), swaps,
+    // scales and eliminates with the same operation per element, so the
+    // solution is the serial one bit for bit.  A zero pivot = singular.
+    bool sing = !(s_tr > 0.0);
+    {
+      const int gj = tid >> 4, gk = tid & 15;
+      const bool own = gj < m && gk < m;
+      unsigned pivoted = 0;  // bit c: column c holds a pivot
       for (int i = 0; i < m && !sing; i++) {
-        double big = 0.0;
-        int irow = 0, icol = 0;
-        for (int j = 0; j < m; j++)
-          if (ipiv[j] != 1)
-            for (int kk = 0; kk < m; kk++)
-              if (ipiv[kk] == 0 && fabs(U[j * m + kk]) >= big) {
-                big = fabs(U[j * m + kk]);
-                irow = j;
-                icol = kk;
-              }
-        ++ipiv[icol];
-        if (irow != icol) {
-          for (int l = 0; l < m; l++) {
-            double t = U[irow * m + l];
-            U[irow * m + l] = U[icol * m + l];
-            U[icol * m + l] = t;
-          }
-          double t = bvec[irow];
-          bvec[irow] = bvec[icol];
-          bvec[icol] = t;
+        double v = -2.0;
+        int idx = -1;
+        if (own && !((pivoted >> gj) & 1u) && !((pivoted >> gk) & 1u)) {
+          const double av = fabs(U[gj * m + gk]);
+          v = (av == av) ? av : -1.0;  // a NaN never wins, as in the scan
+          idx = gj * m + gk;
         }
-        indxr[i] = irow;
-        indxc[i] = icol;
-        if (U[icol * m + icol] == 0.0) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+          const double ov = __shfl_xor(v, off);
+          const int oi = __shfl_xor(idx, off);
+          if (ov > v || (ov == v && oi > idx)) {
+            v = ov;
+            idx = oi;
+          }
+        }
+        if ((tid & 63) == 0) {
+          s_pv[tid >> 6] = v;
+          s_pi[tid >> 6] = idx;
+        }
+        __syncthreads();
+        v = s_pv[0];
+        idx = s_pi[0];
+#pragma unroll
+        for (int w2 = 1; w2 < AM_THREADS / 64; w2++)
+          if (s_pv[w2] > v || (s_pv[w2] == v && s_pi[w2] > idx)) {
+            v = s_pv[w2];
+            idx = s_pi[w2];
+          }
+        const int irow = idx / m, icol = idx - (idx / m) * m;
+        pivoted |= 1u << icol;
+        if (irow != icol) {
+          if (tid < m) {
+            const double t = U[irow * m + tid];
+            U[irow * m + tid] = U[icol * m + tid];
+            U[icol * m + tid] = t;
+          }
+          if (tid == 0) {
+            const double t = bvec[irow];
+            bvec[irow] = bvec[icol];
+            bvec[icol] = t;
+          }
+        }
+        __syncthreads();
+        const double piv = U[icol * m + icol];
+        if (piv == 0.0) {
           sing = true;
           break;
         }
-        const double pivinv = 1.0 / U[icol * m + icol];
-        U[icol * m + icol] = 1.0;
-        for (int l = 0; l < m; l++) U[icol * m + l] *= pivinv;
-        bvec[icol] *= pivinv;
-        for (int ll = 0; ll < m; ll++)
-          if (ll != icol) {
-            const double dum = U[ll * m + icol];
-            U[ll * m + icol] = 0.0;
-            for (int l = 0; l < m; l++) U[ll * m + l] -= U[icol * m + l] * dum;
-            bvec[ll] -= bvec[icol] * dum;
-          }
+        const double pivinv = 1.0 / piv;
+        double cur_v = 0.0, dum = 0.0, prow = 0.0, bj = 0.0, dumb = 0.0;
+        if (own) {
+          cur_v = U[gj * m + gk];
+          dum = U[gj * m + icol];
+          prow = U[icol * m + gk];
+        }
+        if (tid < m) {
+          bj = bvec[tid];
+          dumb = U[tid * m + icol];
+        }
+        const double bpiv = bvec[icol];
+        __syncthreads();
+        if (own) {
+          const double pr = (gk == icol ? 1.0 : prow) * pivinv;  // scaled pivot row
+          U[gj * m + gk] = (gj == icol) ? pr : (gk == icol ? 0.0 : cur_v) - pr * dum;
+        }
+        if (tid < m) {
+          const double bp = bpiv * pivinv;
+          bvec[tid] = (tid == icol) ? bp : bj - bp * dumb;
+        }
+        __syncthreads();
       }
-      (void)indxr;
-      (void)indxc;
-      for (int i = 0; i < m && !sing; i++)
-        if (!isfinite(bvec[i])) sing = true;
-      s_m = sing ? 0 : m;
-      if (sing) ms.restarts++;
     }
+    for (int i = 0; i < m && !sing; i++)
+      if (!isfinite(bvec[i])) sing = true;
+    if (sing) ms.restarts++;
+    if (tid == 0) s_m = sing ? 0 : m;
     __syncthreads();
     const int sm = __builtin_amdgcn_readfirstlane(s_m);
     if (sm == 0) stored = 0;  // keep only the current pair
@@ -277,7 +320,8 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     const double rk = hr[(long long)cur * n + u];
     double wb = wk, rb = rk;
     for (int i = 0; i < m; i++) {
-      const int sl = ((cur - (i + 1)) % S + S) % S;
+      int sl = cur - (i + 1);  // ring slot of iterate k-1-i (m < S)
+      sl += (sl < 0) ? S : 0;
       wb -= bvec[i] * (wk - hw[(long long)sl * n + u]);
       rb -= bvec[i] * (rk - hr[(long long)sl * n + u]);
     }
