@@ -39,7 +39,9 @@ __device__ __forceinline__ double mix_residual(double r, double pt, int transfor
 }
 
 __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
-  const int inst = blockIdx.x;
+  // the largest instances are dispatched first, so the small ones fill the
+  // CUs behind them instead of each round waiting on a large one
+  const int inst = a.order ? a.order[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int tid = threadIdx.x;
   __shared__ double U[AM_MAXH * AM_MAXH];
   __shared__ double bvec[AM_MAXH];
@@ -145,22 +147,33 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // push (w_k, g_k) into slot `cur`
   const double* __restrict__ xn = a.x + d.xo;
   const bool pc = a.precond > 0.0 && n > 1;
-  for (int u = tid; u < n; u += AM_THREADS) {
-    hw[(long long)cur * n + u] = w[u];
-    double f = mix_residual(r[u], pt[u], a.transform, a.floor);
+  // (whole-wave trips: the neighbours' mixing residuals come from the lanes
+  // beside, and only a wave's edge lanes evaluate theirs again)
+  for (int u0 = 0; u0 < n; u0 += AM_THREADS) {
+    const int u = u0 + tid, lane = tid & 63;
+    const bool act = u < n;
+    double f = act ? mix_residual(r[u], pt[u], a.transform, a.floor) : 0.0;
     if (pc) {
-      const int i = u + 1;  // node of unknown u
-      const bool has_l = u > 0, has_r = u + 1 < n;
-      const double fl = has_l ? mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor) : 0.0;
-      const double fr = has_r ? mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor) : 0.0;
-      const double hl = has_l ? xn[i] - xn[i - 1] : 0.0;
-      const double hrr = has_r ? xn[i + 1] - xn[i] : 0.0;
-      const double mm = 0.5 * (hl + hrr);
-      const double cl = has_l ? (1.0 / hl) / mm : 0.0;
-      const double cr = has_r ? (1.0 / hrr) / mm : 0.0;
-      f = f + a.precond * ((cl + cr) * f - cl * fl - cr * fr);
+      double fl = __shfl_up(f, 1), fr = __shfl_down(f, 1);
+      if (act) {
+        const int i = u + 1;  // node of unknown u
+        const bool has_l = u > 0, has_r = u + 1 < n;
+        if (lane == 0 && has_l) fl = mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor);
+        if (lane == 63 && has_r) fr = mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor);
+        fl = has_l ? fl : 0.0;
+        fr = has_r ? fr : 0.0;
+        const double hl = has_l ? xn[i] - xn[i - 1] : 0.0;
+        const double hrr = has_r ? xn[i + 1] - xn[i] : 0.0;
+        const double mm = 0.5 * (hl + hrr);
+        const double cl = has_l ? (1.0 / hl) / mm : 0.0;
+        const double cr = has_r ? (1.0 / hrr) / mm : 0.0;
+        f = f + a.precond * ((cl + cr) * f - cl * fl - cr * fr);
+      }
     }
-    hr[(long long)cur * n + u] = f;
+    if (act) {
+      hw[(long long)cur * n + u] = w[u];
+      hr[(long long)cur * n + u] = f;
+    }
   }
   __syncthreads();
 
@@ -184,15 +197,26 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     const int per = ((n + 4 * NW - 1) / (4 * NW)) * 4;
     const int ub = wave * per, ue = min(n, ub + per);
     d4 accG = {0.0, 0.0, 0.0, 0.0}, accB = {0.0, 0.0, 0.0, 0.0};
-    for (int u0 = ub; u0 < ue; u0 += 4) {
-      const int u = u0 + ku;
-      const bool ok = u < ue;
-      const int uc = ok ? u : ub;  // in-range index for the masked lanes
-      const double rk = ok ? hk[uc] : 0.0;
-      const double hv = hi[uc];
-      const double dv = (ok && use) ? rk - hv : 0.0;
-      accG = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, dv, accG, 0, 0, 0);
-      accB = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, vi == 0 ? rk : 0.0, accB, 0, 0, 0);
+    // GU blocks of 4 unknowns per trip: their loads are all in flight before
+    // the first product (blocks past the range multiply zeros)
+    constexpr int GU = 8;
+    for (int u0 = ub; u0 < ue; u0 += 4 * GU) {
+      double rkv[GU], hvv[GU];
+#pragma unroll
+      for (int g = 0; g < GU; g++) {
+        const int u = u0 + 4 * g + ku;
+        const int uc = (u < ue) ? u : ub;  // in-range index for the masked lanes
+        rkv[g] = hk[uc];
+        hvv[g] = hi[uc];
+      }
+#pragma unroll
+      for (int g = 0; g < GU; g++) {
+        const bool ok = u0 + 4 * g + ku < ue;
+        const double rk = ok ? rkv[g] : 0.0;
+        const double dv = (ok && use) ? rk - hvv[g] : 0.0;
+        accG = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, dv, accG, 0, 0, 0);
+        accB = __builtin_amdgcn_mfma_f64_16x16x4f64(dv, vi == 0 ? rk : 0.0, accB, 0, 0, 0);
+      }
     }
     // result layout: lane l, element e holds row l/16 + 4 e, column l%16
 #pragma unroll

@@ -116,6 +116,7 @@ struct scft_batch {
   double *d_hw = nullptr, *d_hr = nullptr, *d_wcold = nullptr;
   MixState* d_ms = nullptr;
   int* d_iters = nullptr;
+  int* d_mix_order = nullptr;  // instances by decreasing n (mixer dispatch order)
   // algorithmic work per full sweep (all instances)
   double node_steps_all = 0.0, bytes_all = 0.0;
   bool has_wcold = false;
@@ -452,6 +453,7 @@ extern "C" int scft_batch_destroy(scft_batch* b) {
   dfree(b->d_wcold);
   dfree(b->d_ms);
   dfree(b->d_iters);
+  dfree(b->d_mix_order);
   if (b->h_active) (void)hipHostFree(b->h_active);
   for (int s = 0; s < scft_batch::kSide; s++) {
     if (b->side[s]) (void)hipStreamSynchronize(b->side[s]);
@@ -710,6 +712,7 @@ static int ensure_mixer(scft_batch* b, int S) {
   dfree(b->d_hr);
   dfree(b->d_ms);
   dfree(b->d_iters);
+  dfree(b->d_mix_order);
   long long off = 0;
   for (auto& d : b->desc) {
     d.ho = off;
@@ -720,6 +723,13 @@ static int ensure_mixer(scft_batch* b, int S) {
   SCFT_TRY(dalloc(&b->d_hr, off));
   SCFT_TRY(dalloc(&b->d_ms, b->n_inst));
   SCFT_TRY(dalloc(&b->d_iters, b->n_inst));
+  std::vector<int> order(b->n_inst);
+  for (int k = 0; k < b->n_inst; k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int p, int q) { return b->desc[p].n > b->desc[q].n; });
+  SCFT_TRY(dalloc(&b->d_mix_order, b->n_inst));
+  HIPCHK(hipMemcpy(b->d_mix_order, order.data(), sizeof(int) * b->n_inst,
+                   hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_desc, b->desc.data(), sizeof(InstDesc) * b->n_inst,
                    hipMemcpyHostToDevice));
   b->hist_S = S;
@@ -752,6 +762,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   const int nb = static_cast<int>(b->buckets.size());
   AndersonArgs a;
   std::memset(&a, 0, sizeof a);  // padding included: the struct is compared bytewise
+  a.order = b->d_mix_order;
   a.desc = b->d_desc;
   a.w = b->d_w;
   a.r = b->d_r;

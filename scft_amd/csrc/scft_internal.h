@@ -61,6 +61,7 @@ struct MixState {
 };
 
 struct AndersonArgs {
+  const int* order;  // instance of workgroup g: largest n first (nullable: g)
   const InstDesc* desc;
   double* w;            // field on unknowns (updated in place)
   const double* r;      // plain residual phi - phi_t from the march
