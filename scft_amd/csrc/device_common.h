@@ -7,6 +7,14 @@ namespace scft {
 
 __device__ __forceinline__ void wg_barrier() { __syncthreads(); }
 
+// Wait until every vector-memory operation of this wave has completed
+// (s_waitcnt vmcnt(0); gfx9 encoding: vmcnt in bits 3:0 and 15:14, expcnt in
+// 6:4 and lgkmcnt in 11:8 left at their maxima).  Issued once before a
+// software-pipelined load loop, it leaves only the loop's own loads in
+// flight.  The compiler's per-step waits then count exactly the loads newer
+// than the one being consumed, instead of draining the queue every trip.
+__device__ __forceinline__ void vmem_drain() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // DPP lane moves on a double (two 32-bit halves).  CTRL is a gfx9 DPP
 // control: 0x110 + s = row_shr:s (lane p reads p - s of its 16-lane row),
 // 0x100 + s = row_shl:s (reads p + s).  Lanes whose source leaves the row

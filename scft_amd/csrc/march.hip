@@ -467,8 +467,16 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
     for (int t = 0; t < K; t++)
       dstv[t] = slab_load(srsrc, off_src, static_cast<unsigned>((max(s, 0) * (P * K) + t * P) * 8));
   };
+  // The loop below waits on each ring set by the count of loads issued after
+  // it.  That count is exact only if nothing else is in flight (the first
+  // half's slab stores are drained here) and the sets are issued in ring
+  // order (the scheduler may not reorder them across the barrier).
+  vmem_drain();
 #pragma unroll
-  for (int e = 0; e < PD; e++) fetch(half - 1 - e, ring[e]);
+  for (int e = 0; e < PD; e++) {
+    fetch(half - 1 - e, ring[e]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
   for (int i0 = 1; i0 <= half; i0 += PD) {
 #pragma unroll
     for (int e = 0; e < PD; e++) {
