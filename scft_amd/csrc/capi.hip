@@ -677,9 +677,10 @@ extern "C" int scft_debug_slabs(scft_batch* b, int inst, double* out) {
                      hipMemcpyDeviceToHost));
     for (int s = 1; s < half; s++)
       for (int u = 0; u < d.n; u++) {
-        const int p = u / K, t = u % K;
-        out[static_cast<size_t>(s - 1) * d.n + u] =
-            raw[static_cast<size_t>(s) * P * K + static_cast<size_t>(t) * P + p];
+        const int p = u / K, t = u % K;  // lane p, row t (slab_put's layout)
+        const size_t e = K >= 2 ? static_cast<size_t>(t / 2) * 2 * P + 2 * p + t % 2
+                                : static_cast<size_t>(p);
+        out[static_cast<size_t>(s - 1) * d.n + u] = raw[static_cast<size_t>(s) * P * K + e];
       }
     return SCFT_OK;
   }

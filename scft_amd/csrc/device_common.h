@@ -92,6 +92,49 @@ __device__ __forceinline__ double slab_load(__amdgpu_buffer_rsrc_t r, unsigned l
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, lane_off, uni_off, 2));
 }
 
+// One slab holds the K rows of each of the P lanes of a workgroup after one
+// step.  For K >= 2 the rows go in pairs: rows 2i and 2i+1 of lane p sit at
+// i * 2P + 2p and the double after it, so a wave moves 16 contiguous bytes
+// per lane (1 KB per instruction).  For K = 1 it is one row of P.
+//   element (row t, lane p):  K >= 2: (t / 2) * 2P + 2p + t % 2;  K = 1: p.
+// lane_off = p * slab_lane_bytes<K>(); slab_off = byte offset of the slab.
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+template <int K>
+__device__ constexpr unsigned slab_lane_bytes() {
+  return K >= 2 ? 16u : 8u;
+}
+template <int K>
+__device__ __forceinline__ void slab_put(const double (&q)[K], __amdgpu_buffer_rsrc_t r,
+                                         unsigned lane_off, unsigned slab_off, int P) {
+  if constexpr (K >= 2) {
+#pragma unroll
+    for (int t = 0; t < K; t += 2) {
+      const f64x2 v = {q[t], q[t + 1]};
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, v), r, lane_off,
+                                             slab_off + static_cast<unsigned>(t * P * 8), 2);
+    }
+  } else {
+    slab_store(q[0], r, lane_off, slab_off);
+  }
+}
+template <int K>
+__device__ __forceinline__ void slab_get(double (&d)[K], __amdgpu_buffer_rsrc_t r,
+                                         unsigned lane_off, unsigned slab_off, int P) {
+  if constexpr (K >= 2) {
+#pragma unroll
+    for (int t = 0; t < K; t += 2) {
+      const f64x2 v = __builtin_bit_cast(
+          f64x2, __builtin_amdgcn_raw_buffer_load_b128(
+                     r, lane_off, slab_off + static_cast<unsigned>(t * P * 8), 2));
+      d[t] = v.x;
+      d[t + 1] = v.y;
+    }
+  } else {
+    d[0] = slab_load(r, lane_off, slab_off);
+  }
+}
+
 // ---- double-double arithmetic (hi + lo, |lo| <= ulp(hi)/2) for the pivot
 // scan.  The pivots are a projective product of 2x2 matrices whose two
 // eigenvalues nearly coincide when diffusion dominates (dt/h^2 >> 1); in FP64

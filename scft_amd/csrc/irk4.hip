@@ -467,14 +467,10 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
   const bool lane_live = u0 < n;
   const __amdgpu_buffer_rsrc_t srsrc =
       slab_rsrc(slab, static_cast<unsigned>(a.slab_stride * sizeof(double)));
-  const unsigned off_dst = static_cast<unsigned>(tid) * 8u;
+  const unsigned off_dst = static_cast<unsigned>(tid) * slab_lane_bytes<K>();
   const unsigned off_src = lane_live ? off_dst : 0u;
   auto store_slab = [&](const int j) __attribute__((always_inline)) {
-    if (lane_live) {
-#pragma unroll
-      for (int t = 0; t < K; t++)
-        slab_store(q[t], srsrc, off_dst, static_cast<unsigned>((j * (P * K) + t * P) * 8));
-    }
+    if (lane_live) slab_put<K>(q, srsrc, off_dst, static_cast<unsigned>(j * (P * K) * 8), P);
   };
   store_slab(0);
   for (int j = 1; j < half; j++) {
@@ -490,9 +486,7 @@ __device__ void irk4_one(const MarchArgs& a, const int inst, const unsigned st0,
   constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
   double ring[PD][K];
   auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int t = 0; t < K; t++)
-      dstv[t] = slab_load(srsrc, off_src, static_cast<unsigned>((max(s, 0) * (P * K) + t * P) * 8));
+    slab_get<K>(dstv, srsrc, off_src, static_cast<unsigned>(max(s, 0) * (P * K) * 8), P);
   };
   // The loop below waits on each ring set by the count of loads issued after
   // it.  That count is exact only if nothing else is in flight (the first

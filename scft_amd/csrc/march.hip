@@ -424,7 +424,9 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   };
 
   // Slab s (q after step s, s = 0 .. N/2 - 1; slab 0 is the initial
-  // condition) sits at s * P * K + t * P + tid.  Lanes whose rows are all
+  // condition) starts at s * P * K; within it row t of lane tid sits at
+  // (t / 2) * 2P + 2 tid + t % 2 (K >= 2: 16 bytes per lane and access) or at
+  // tid (K = 1).  Lanes whose rows are all
   // padding (u0 >= n) store nothing (one exec mask for the whole loop) and
   // load from lane 0's address, so a wave's dead lanes share one cache line;
   // what they load only ever multiplies their q = 0.  The region is accessed
@@ -434,14 +436,10 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   const bool lane_live = u0 < n;
   const __amdgpu_buffer_rsrc_t srsrc =
       slab_rsrc(slab, static_cast<unsigned>(a.slab_stride * sizeof(double)));
-  const unsigned off_dst = static_cast<unsigned>(tid) * 8u;
+  const unsigned off_dst = static_cast<unsigned>(tid) * slab_lane_bytes<K>();
   const unsigned off_src = lane_live ? off_dst : 0u;
   auto store_slab = [&](const int j) __attribute__((always_inline)) {
-    if (lane_live) {
-#pragma unroll
-      for (int t = 0; t < K; t++)
-        slab_store(q[t], srsrc, off_dst, static_cast<unsigned>((j * (P * K) + t * P) * 8));
-    }
+    if (lane_live) slab_put<K>(q, srsrc, off_dst, static_cast<unsigned>(j * (P * K) * 8), P);
   };
   store_slab(0);
   for (int j = 1; j < half; j++) {
@@ -463,9 +461,7 @@ __device__ void march_one(const MarchArgs& a, const int inst, const unsigned st0
   constexpr int PD = (K <= 2) ? 8 : (K == 4) ? 4 : 2;
   double ring[PD][K];
   auto fetch = [&](const int s, double (&dstv)[K]) __attribute__((always_inline)) {
-#pragma unroll
-    for (int t = 0; t < K; t++)
-      dstv[t] = slab_load(srsrc, off_src, static_cast<unsigned>((max(s, 0) * (P * K) + t * P) * 8));
+    slab_get<K>(dstv, srsrc, off_src, static_cast<unsigned>(max(s, 0) * (P * K) * 8), P);
   };
   // The loop below waits on each ring set by the count of loads issued after
   // it.  That count is exact only if nothing else is in flight (the first
