@@ -64,9 +64,14 @@ SOLVER_TEXT = ("device Anderson, history 16, log mixing preconditioned by I + 0.
 
 
 def march_source_sha256():
+    """Hash of the march kernel's sources (the kernel and the device helpers
+    it inlines: slab layout and access, lane moves)."""
     import hashlib
-    with open(os.path.join(ROOT, "scft_amd", "csrc", "march.hip"), "rb") as f:
-        return hashlib.sha256(f.read()).hexdigest()
+    h = hashlib.sha256()
+    for name in ("march.hip", "device_common.h"):
+        with open(os.path.join(ROOT, "scft_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()
 
 
 def measured_traffic(kern, n_items, alg_bytes):
