@@ -218,8 +218,19 @@ int scft_batch_time_buckets(scft_batch *b, const double *w, int reps,
                             double *bytes, int *n_items);
 
 /* Autotuner: time every (nodes-per-lane, waves) variant that fits each size
- * bucket and keep the fastest; written to desc.tune_file when given. */
+ * bucket and keep the fastest; written to desc.tune_file when given.  Ends
+ * with scft_batch_tune_placement. */
 int scft_batch_autotune(scft_batch *b, int reps);
+/* No counterpart in the reference.  The march's speed depends on which
+ * physical pages back the slab scratch (on MI355X the same launch ran 20-25%
+ * faster on some allocations than on others, whatever the virtual address).
+ * Tries up to `tries` further allocations of the scratch, each made while the
+ * earlier ones are held so that it gets other pages, times a whole sweep of
+ * the current device field on each (`reps` sweeps) and keeps the fastest;
+ * the others are freed.  Stops early when free device memory would fall
+ * below a quarter of the scratch plus 1 GiB.  Cannot be stored in a tune
+ * file: call it after creating (or re-tuning) a batch. */
+int scft_batch_tune_placement(scft_batch *b, int tries, int reps);
 int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
                       int *K, int *W, int max_buckets);
 

@@ -118,6 +118,7 @@ def _load():
     L.scft_batch_time_buckets.argtypes = [C.c_void_p, _dp, C.c_int, C.c_int,
                                           C.POINTER(C.c_int), _dp, _dp, _ip]
     L.scft_batch_autotune.argtypes = [C.c_void_p, C.c_int]
+    L.scft_batch_tune_placement.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.scft_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), _ip, _ip, _ip, C.c_int]
     L.scft_debug_slabs.argtypes = [C.c_void_p, C.c_int, _dp]
     L.scft_batch_set_walls.argtypes = [C.c_void_p, _dp, _dp]
@@ -334,6 +335,11 @@ class Batch:
 
     def autotune(self, reps=2):
         check(lib.scft_batch_autotune(self.h, reps), "scft_batch_autotune")
+
+    def tune_placement(self, tries=3, reps=2):
+        """Keep the fastest of 1 + tries allocations of the slab scratch
+        (scft_batch_tune_placement); autotune() ends with it."""
+        check(lib.scft_batch_tune_placement(self.h, tries, reps), "scft_batch_tune_placement")
 
     def tuning(self):
         nb = C.c_int()

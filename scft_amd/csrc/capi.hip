@@ -587,6 +587,61 @@ extern "C" int scft_batch_time_buckets(scft_batch* b, const double* w, int reps,
   return SCFT_OK;
 }
 
+// Wall time of `reps` whole sweeps of the current device field (HIP events
+// on the main stream, after one untimed sweep).
+static int time_sweeps(scft_batch* b, int reps, float* ms) {
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int rc = enqueue_sweep(b);
+  if (rc == SCFT_OK && hipEventRecord(e0, b->stream) != hipSuccess) rc = SCFT_E_HIP;
+  for (int i = 0; i < reps && rc == SCFT_OK; i++) rc = enqueue_sweep(b);
+  if (rc == SCFT_OK && (hipEventRecord(e1, b->stream) != hipSuccess ||
+                        hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+    rc = SCFT_E_HIP;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == SCFT_OK) *ms /= static_cast<float>(reps);
+  return rc;
+}
+
+extern "C" int scft_batch_tune_placement(scft_batch* b, int tries, int reps) {
+  if (!b || !b->d_slabs || tries < 0) return SCFT_E_ARG;
+  if (reps < 1) reps = 1;
+  HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
+  const size_t bytes = static_cast<size_t>(b->slab_doubles) * sizeof(double);
+  std::vector<double*> cand{b->d_slabs};
+  float best_ms = 0.f;
+  int best = 0, rc = time_sweeps(b, reps, &best_ms);
+  for (int c = 0; c < tries && rc == SCFT_OK; c++) {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess ||
+        free_b < bytes + bytes / 4 + (size_t(1) << 30))
+      break;
+    double* p = nullptr;
+    if (hipMalloc(reinterpret_cast<void**>(&p), bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      break;
+    }
+    cand.push_back(p);
+    b->d_slabs = p;
+    drop_sweep_graph(b);
+    float ms = 0.f;
+    rc = time_sweeps(b, reps, &ms);
+    if (rc == SCFT_OK && ms < best_ms) {
+      best_ms = ms;
+      best = static_cast<int>(cand.size()) - 1;
+    }
+  }
+  (void)hipStreamSynchronize(b->stream);
+  for (size_t i = 0; i < cand.size(); i++)
+    if (static_cast<int>(i) != best) (void)hipFree(cand[i]);
+  b->d_slabs = cand[best];
+  drop_sweep_graph(b);
+  return rc;
+}
+
 extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
   if (!b) return SCFT_E_STATE;
   if (reps < 1) reps = 1;
@@ -643,7 +698,7 @@ extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
   (void)hipEventDestroy(e1);
   SCFT_TRY(size_scratch(b));
   write_tune_file(b);
-  return SCFT_OK;
+  return scft_batch_tune_placement(b, 3, std::max(2, reps));
 }
 
 extern "C" int scft_batch_tuning(const scft_batch* b, int* n_buckets, int* bucket_nmax, int* K,
