@@ -336,7 +336,7 @@ class Batch:
     def autotune(self, reps=2):
         check(lib.scft_batch_autotune(self.h, reps), "scft_batch_autotune")
 
-    def tune_placement(self, tries=3, reps=2):
+    def tune_placement(self, tries=5, reps=2):
         """Keep the fastest of 1 + tries allocations of the slab scratch
         (scft_batch_tune_placement); autotune() ends with it."""
         check(lib.scft_batch_tune_placement(self.h, tries, reps), "scft_batch_tune_placement")

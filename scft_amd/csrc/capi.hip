@@ -698,7 +698,7 @@ extern "C" int scft_batch_autotune(scft_batch* b, int reps) {
   (void)hipEventDestroy(e1);
   SCFT_TRY(size_scratch(b));
   write_tune_file(b);
-  return scft_batch_tune_placement(b, 3, std::max(2, reps));
+  return scft_batch_tune_placement(b, 5, std::max(2, reps));
 }
 
 extern "C" int scft_batch_tuning(const scft_batch* b, int* n_buckets, int* bucket_nmax, int* K,

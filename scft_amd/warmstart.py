@@ -153,8 +153,13 @@ class Prepared:
         self.F = features(S, xs, cfg["Ns"], cfg["w0"]).astype(np.float32)
         self.grids = grid(np.array([x[-1] for x in xs], dtype=np.float64))
         self.xu, self.nu, self.uoff = S._tables([x[1:] for x in xs])
+        self.gx, self.gn, self.goff = S._tables(self.grids)
         ks = [spike_nodes(x) for x in xs]
         self.spikes = [(i, int(self.uoff[i]), k) for i, k in enumerate(ks) if k > 0]
+        # the node samples as one scatter: out[sp_dst] = spike[sp_inst, sp_node]
+        self.sp_inst = np.concatenate([np.full(k, i) for i, _, k in self.spikes] or [[]]).astype(np.int64)
+        self.sp_node = np.concatenate([np.arange(k) for _, _, k in self.spikes] or [[]]).astype(np.int64)
+        self.sp_dst = np.concatenate([o + np.arange(k) for _, o, k in self.spikes] or [[]]).astype(np.int64)
 
 
 class WarmStart:
@@ -178,14 +183,13 @@ class WarmStart:
         layout: one network forward and one batched resampling call."""
         S = self.S
         Y = self.mlp.forward(prep.F).astype(np.float64)
-        ys, _, _ = S._tables(Y[:, :GRID])
-        gx, gn, goff = S._tables(prep.grids)
+        ys = np.ascontiguousarray(Y[:, :GRID]).reshape(-1)
         out = np.empty(int(prep.nu.sum()))
-        S.check(S.lib.scft_resample_spline_batch(len(gn), gn, goff, gx, ys, prep.nu, prep.uoff,
-                                                 prep.xu, out), "scft_resample_spline_batch")
+        S.check(S.lib.scft_resample_spline_batch(len(prep.gn), prep.gn, prep.goff, prep.gx, ys,
+                                                 prep.nu, prep.uoff, prep.xu, out),
+                "scft_resample_spline_batch")
         spike = W_SCALE * np.sinh(Y[:, GRID:] @ _NODE_W.T)
-        for i, o, k in prep.spikes:
-            out[o:o + k] = spike[i, :k]
+        out[prep.sp_dst] = spike[prep.sp_inst, prep.sp_node]
         return out
 
     def close(self):
