@@ -165,6 +165,33 @@ def solver_opts(S, max_iter):
                            progress_seconds=20.0, **SOLVER)
 
 
+def stepper_compare(args):
+    """Sweep time of 4096 copies of the field the reference's deal.II IRK
+    solver converged (DEALII_SCFT/inputFiles/N=33_for_read.txt, consistent
+    w-mass, wall values from the file), implicit Euler against IRK4, with the
+    residual each leaves on that field (the discretisation error of phi)."""
+    import re
+    import scft_amd as S
+    from scft_amd import workloads as WL
+    path = os.path.join(ROOT, "tests", "golden", "files", "N=33_for_read.txt")
+    rows = [l for l in open(path) if re.match(r"^\s*\d+\s*,", l)]
+    d = np.array([[float(v) for v in r.split(",")] for r in rows])
+    x, w = d[:, 1], d[:, 2]
+    pt = WL.exp_target(np.minimum(x, x[-1] - x))  # both walls
+    n = args.n_inst
+    for stepper, N in [(S.STEP_IE, 4096), (S.STEP_IE, 2048), (S.STEP_IRK4, 256), (S.STEP_IRK4, 512)]:
+        b = S.Batch([x] * n, [N] * n, [pt[1:-1]] * n, bc=S.BC_DD, cmat=S.C_CONSISTENT,
+                    stepper=stepper)
+        b.set_walls([w[0]] * n, [w[-1]] * n)
+        b.upload([w[1:-1]] * n)
+        b.autotune(reps=2)
+        ms = b.time_eval([w[1:-1]] * n, reps=10)
+        out = b.eval([w[1:-1]] * n)
+        b.close()
+        log(f"bench: {'IE' if stepper == S.STEP_IE else 'IRK4'} N={N}: {ms:.3f} ms per sweep of "
+            f"{n} instances, ||r|| of the reference field {out['rnorm'][0]:.2e}")
+
+
 def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
     """The CPU oracle (same mixer, options and tolerance) solving a random
     sample of the batch on `threads` OpenMP threads.  The sample is drawn
@@ -220,11 +247,18 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB 5 times and exit (counter scale)")
     ap.add_argument("--tune-file", default=None)
+    ap.add_argument("--stepper-compare", action="store_true",
+                    help="time one sweep of 4096 copies of the reference's 33-node deal.II "
+                         "problem with implicit Euler and with IRK4 (no JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (--gpus {args.gpus} requested): instances are "
               "independent, so a multi-GPU run is N copies of this one", file=sys.stderr)
         sys.exit(2)
+
+    if args.stepper_compare:
+        stepper_compare(args)
+        return
 
     if args.copy_calibration:
         # a plain copy kernel of known size: its FETCH_SIZE / WRITE_SIZE per
