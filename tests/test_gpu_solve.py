@@ -52,6 +52,30 @@ def test_device_anderson_trajectory_matches_oracle(transform, beta, cap, precond
     b.close()
 
 
+def test_production_mixer_trajectory_twelve_updates():
+    """The bench's mixer (preconditioned log mixing, history 16, beta 1, no
+    step cap) is far better conditioned than plain mixing: device and oracle
+    stay within ~4e-11 (relative) over 12 updates, so the Gram system and the
+    Gauss-Jordan elimination are compared at history up to 11 (bar 1e-8)."""
+    S = _S()
+    K = 12
+    cfg = WL.config3(n_inst=3, seed=7, Ns=(256,))
+    o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0, max_iter=K + 1,
+                              precond=0.5)
+    o_cpu.max_iter = K
+    b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN)
+    out = b.solve([w[1:] for w in cfg["w0"]], o_dev)
+    for k in range(3):
+        ref = O.solve_anderson(cfg["xs"][k], cfg["pts"][k], cfg["Ns"][k], cfg["w0"][k], o_cpu,
+                               bc=O.BC_DN)
+        dw = np.abs(out["w"][k] - ref["w"][1:]).max() / max(1.0, np.abs(ref["w"]).max())
+        print(f"instance {k}: m={len(cfg['xs'][k]) - 1} relative |w_dev - w_cpu| after {K} "
+              f"updates = {dw:.2e}")
+        assert dw <= 1e-8
+        assert out["iters"][k] == K + 1
+    b.close()
+
+
 def test_device_anderson_converges_config1():
     """Config 1 (m = 32, N = 2048, w0 = 0), history 10, mixing 0.1, cap 0.5:
     converges on the device and the converged field evaluates (on the CPU
