@@ -1,0 +1,169 @@
+"""Pins of the oracle parts that no reference fixture covers.
+
+* The graded mesh of the production mix (oracle/scft_oracle.c:
+  orc_graded_elements, orc_mesh_graded; scft_amd/workloads.py:graded_mesh).
+  The reference has no graded generator; its general-mesh stencils
+  (Matlab_files/simple_FEM_1D_transient.m:35-63) take any node vector.  The
+  specification h(x) = h0 (1 + x/ell) is pinned by what it fixes exactly:
+  h_i / (1 + x_i/ell) is one constant, the end points are exact, the element
+  count is round(ell ln(1 + L/ell) / h0) clamped to [63, 4095], and the nodes
+  solve the recurrence x_{i+1} = x_i + h(x_i) written out independently.
+* The residual-form Anderson mixer (oracle_solvers.c:orc_anderson), which
+  the device mixer follows.  On an affine residual r(w) = b - A w with the
+  history at least n and no regularisation, Anderson mixing is GMRES in
+  disguise (Walker & Ni, SIAM J. Numer. Anal. 49 (2011) 1715-1735, Thm 2.2):
+  the iterate after n+1 evaluations is the exact fixed point, so the
+  (n+2)-th evaluation reads a residual at rounding level.  A wrong sign of
+  the least-squares coefficients, a dropped history column or a transposed
+  Gram system loses that finite termination.  History 0 is plain mixing
+  w + beta r, and the reference's linear toy (1D_FEM.c:372-379, fixed point
+  (-4, 6)) is reached in at most 4 evaluations.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from tests.helpers import workloads
+
+WL = workloads()
+
+
+# ------------------------------------------------------------------ graded mesh
+def _draw(rng):
+    L = rng.uniform(2.0, 8.0)
+    h0 = math.exp(rng.uniform(math.log(1e-4), math.log(1e-2)))
+    ell = rng.uniform(0.05, 0.5)
+    return L, h0, ell
+
+
+def test_graded_mesh_closed_form():
+    rng = np.random.default_rng(20261205)
+    for _ in range(300):
+        L, h0, ell = _draw(rng)
+        m = O.graded_elements(L, h0, ell, 63, 4095)
+        x = O.mesh_graded(m + 1, L, ell)
+        assert x[0] == 0.0 and x[-1] == L and len(x) == m + 1
+        h = np.diff(x)
+        assert (h > 0).all()
+        g = math.log1p(L / ell) / m
+        c = ell * math.expm1(g)
+        # h_i / (1 + x_i/ell) is the same constant c = ell (e^g - 1) everywhere
+        ratio = h / (1.0 + x[:-1] / ell)
+        assert np.abs(ratio / c - 1.0).max() <= 1e-9, (L, h0, ell)
+        # the recurrence x_{i+1} = x_i + c (1 + x_i/ell), summed independently
+        xr = np.empty(m + 1)
+        xr[0] = 0.0
+        for i in range(m):
+            xr[i + 1] = xr[i] + c * (1.0 + xr[i] / ell)
+        assert np.abs(xr - x).max() <= 1e-12 * L
+        # element count: round(ell ln(1+L/ell)/h0) before clamping, so the first
+        # element is h0 up to the rounding of the count (|dm| <= 1/2)
+        m_raw = ell * math.log1p(L / ell) / h0
+        assert m == min(max(int(round(m_raw)), 63), 4095)
+        if 63 < m < 4095:
+            assert abs(c / h0 - 1.0) <= 0.5 / m + g, (c, h0)
+
+
+def test_graded_mesh_clamps():
+    # huge h0: too few elements -> 63; tiny h0: too many -> 4095
+    assert O.graded_elements(2.0, 0.5, 0.5, 63, 4095) == 63
+    assert O.graded_elements(8.0, 1e-6, 0.5, 63, 4095) == 4095
+    x = WL.graded_mesh(8.0, 1e-6, 0.5)
+    assert len(x) == 4096 and x[-1] == 8.0
+
+
+def test_graded_mesh_workloads_match_oracle():
+    """The workload generator (numpy) and the oracle (C libm) build the same
+    nodes to a few ulp (numpy's and libm's expm1 round differently; every
+    parity test hands one array to both sides, so this never enters a bar)."""
+    rng = np.random.default_rng(7)
+    for _ in range(300):
+        L, h0, ell = _draw(rng)
+        xw = WL.graded_mesh(L, h0, ell)
+        m = O.graded_elements(L, h0, ell, 63, 4095)
+        assert len(xw) == m + 1
+        xo = O.mesh_graded(m + 1, L, ell)
+        assert xw[0] == xo[0] == 0.0 and xw[-1] == xo[-1] == L
+        assert (np.abs(xw - xo) <= 16.0 * np.spacing(np.maximum(np.abs(xo), 1e-300))).all()
+
+
+# ------------------------------------------------------------------ Anderson
+def _affine_solve(A, b, w0, history, beta, max_iter, transform=O.MIX_PLAIN):
+    n = len(w0)
+    calls = []
+
+    def F(nn, w, r, ctx):
+        wv = np.ctypeslib.as_array(w, shape=(nn,))
+        rv = b - A @ wv
+        for i in range(nn):
+            r[i] = rv[i]
+        calls.append(float(np.abs(rv).max()))
+        return 0
+
+    cb = O.RESIDUAL_FN(F)
+    opts = O.anderson_opts(history=history, beta=beta, tol=0.0, max_iter=max_iter,
+                           step_cap=0.0, regularization=0.0, transform=transform)
+    st = O.AndersonStats(0, 0, 0.0, None, 0)
+    w = np.ascontiguousarray(w0, dtype=np.float64).copy()
+    O.lib().orc_anderson(w, n, C.cast(cb, C.c_void_p), None, C.byref(opts), None, C.byref(st))
+    return w, calls
+
+
+@pytest.mark.parametrize("n,beta", [(2, 1.0), (5, 1.0), (9, 1.0), (16, 1.0),
+                                    (2, 0.3), (5, 0.3), (9, 0.3), (12, 0.3)])
+def test_anderson_finite_termination_on_affine_map(n, beta):
+    """GMRES reaches x* in n steps, so evaluation n+2 (index n+1) reads a zero
+    residual.  With beta < 1 the differences of the residuals are shorter and
+    the normal equations of the Gram system (condition squared) leave that
+    step at ~1e-2 on these draws; the next update lands, so one evaluation
+    more is allowed there."""
+    rng = np.random.default_rng(100 + n)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.geomspace(0.1, 3.0, n)) @ Q.T + 0.2 * rng.standard_normal((n, n)) / n
+    b = rng.standard_normal(n)
+    wstar = np.linalg.solve(A, b)
+    last = n + 1 if beta == 1.0 else n + 2
+    w, calls = _affine_solve(A, b, np.zeros(n), history=16, beta=beta, max_iter=last + 1)
+    scale = np.abs(b).max()
+    assert calls[last] <= 1e-9 * scale, calls
+    assert np.abs(w - wstar).max() <= 1e-9 * np.abs(wstar).max()
+    # halfway the residual is far from rounding level (no early accident)
+    assert calls[n // 2] > 1e-6 * scale
+
+
+def test_anderson_truncated_history_does_not_terminate():
+    """Finite termination needs the whole history: with history 2 on n = 9
+    the (n+2)-th residual is still far from zero (the bound above is sharp
+    enough to catch a dropped column)."""
+    n = 9
+    rng = np.random.default_rng(109)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.linspace(0.5, 3.0, n)) @ Q.T
+    b = rng.standard_normal(n)
+    _, calls = _affine_solve(A, b, np.zeros(n), history=2, beta=1.0, max_iter=n + 2)
+    assert calls[n + 1] > 1e-6 * np.abs(b).max()
+
+
+def test_anderson_history_zero_is_plain_mixing():
+    n = 6
+    rng = np.random.default_rng(3)
+    A = np.eye(n) + 0.1 * rng.standard_normal((n, n))
+    b = rng.standard_normal(n)
+    w = np.zeros(n)
+    for _ in range(5):
+        w = w + 0.4 * (b - A @ w)
+    wm, _ = _affine_solve(A, b, np.zeros(n), history=0, beta=0.4, max_iter=5)
+    np.testing.assert_allclose(wm, w, rtol=0, atol=1e-14)
+
+
+def test_anderson_reference_linear_toy():
+    """1D_FEM.c:372-379: x -> (x1/2 - 2, x2/2 + 3); in residual form
+    r = g(x) - x = b - A x with A = I/2, b = (-2, 3).  Fixed point (-4, 6)."""
+    A = 0.5 * np.eye(2)
+    b = np.array([-2.0, 3.0])
+    w, calls = _affine_solve(A, b, np.array([1.0, 2.0]), history=10, beta=0.1, max_iter=4)
+    assert calls[-1] <= 1e-13
+    np.testing.assert_allclose(w, [-4.0, 6.0], rtol=0, atol=1e-12)
