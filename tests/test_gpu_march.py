@@ -3,6 +3,9 @@
 Bar (FP64 throughout):
   * propagator, per stored step j:  ||q_dev,j - q_cpu,j||_inf <= 1e-10 ||q_cpu,j||_inf
   * density and residual:           |phi_dev - phi_cpu| <= 1e-10 (phi is O(1))
+  * next to the wall, entry by entry: the first 8 unknowns of every stored
+    step and of phi within 1e-10 relative (phi_t there is down to ~1e-7 on
+    the graded meshes, far below the norm-wise bars)
 Cases cover both boundary conditions, both w-mass variants, uniform and graded
 meshes, the lane/wave padding edges around 64 K W, and fields from the
 reference's own fixtures.
@@ -17,6 +20,8 @@ pytestmark = pytest.mark.gpu
 
 STEP_RTOL = 1e-10
 PHI_ATOL = 1e-10
+WALL_RTOL = 1e-10
+WALL_NODES = 8
 
 
 def _scft():
@@ -44,6 +49,10 @@ def _cases():
     c.append(("consistent_m100_DD", 100, 256, 3.0, O.BC_DD, O.C_CONSISTENT, None, None))
     c.append(("consistent_m1000_DN", 1000, 512, 4.0, O.BC_DN, O.C_CONSISTENT, None, None))
     c.append(("graded_DN", None, 1024, 5.0, O.BC_DN, O.C_ROWSCALED, (1e-3, 0.2), None))
+    # the production mix's finest wall cells: h0 = 1e-4, N = 8192 (dt/h0^2 ~ 1.2e4)
+    c.append(("graded_h1e-4_N8192_DN", None, 8192, 5.0, O.BC_DN, O.C_ROWSCALED, (1e-4, 0.05),
+              None))
+    c.append(("graded_h3e-4_DD", None, 2048, 3.0, O.BC_DD, O.C_ROWSCALED, (3e-4, 0.3), None))
     c.append(("rough_w_m300", 300, 1024, 3.5, O.BC_DN, O.C_ROWSCALED, None,
               rng.uniform(-5.0, 20.0, 301)))
     c.append(("steps_4096", 200, 4096, 3.0, O.BC_DN, O.C_ROWSCALED, None, None))
@@ -84,7 +93,13 @@ def test_march_matches_oracle(case):
         qc = qc[:, 1:-1] if bc == O.BC_DD else qc[:, 1:]
         err = np.abs(slabs - qc).max(axis=1) / np.abs(qc).max(axis=1)
         assert err.max() <= STEP_RTOL, f"step {err.argmax() + 1}: {err.max():.3e}"
+        k = min(WALL_NODES, qc.shape[1])
+        werr = np.abs(slabs[:, :k] - qc[:, :k]) / np.abs(qc[:, :k])
+        assert werr.max() <= WALL_RTOL, f"wall entry {np.unravel_index(werr.argmax(), werr.shape)}: {werr.max():.3e}"
     phi_ref = unknowns(ref["phi"], bc)
+    k = min(WALL_NODES, len(phi_ref))
+    pwerr = np.abs(out["phi"][0][:k] - phi_ref[:k]) / np.abs(phi_ref[:k])
+    assert pwerr.max() <= WALL_RTOL, f"phi wall entry {pwerr.argmax()}: {pwerr.max():.3e}"
     np.testing.assert_allclose(out["phi"][0], phi_ref, rtol=0, atol=PHI_ATOL)
     np.testing.assert_allclose(out["r"][0], unknowns(ref["r"], bc), rtol=0, atol=PHI_ATOL)
     assert abs(out["rnorm"][0] - ref["rnorm"]) <= PHI_ATOL
@@ -178,6 +193,35 @@ def test_dropin_evaluator_matches_oracle():
     S.lib.scft_adm_chen_f(n, np.ascontiguousarray(w_nr[1:]), o2)
     np.testing.assert_allclose(o2, -ref["r"][1:-1], rtol=0, atol=PHI_ATOL)
     S.finalize()
+
+
+def test_dropin_consistent_extrapolates_walls():
+    """The drop-in evaluator with the consistent w-mass gives every field its
+    wall values by a natural spline through the unknowns, as the deal.II
+    evaluator does (scft.cc:454-480): both walls under DD, the wall at x = 0
+    under DN (the far end is an unknown there)."""
+    from scipy.interpolate import CubicSpline
+    S = _scft()
+    for bc in (S.BC_DD, S.BC_DN):
+        m, N, L = 40, 1024, O.L_REF
+        S.init(N=N, L=L, bc=bc, cmat=S.C_CONSISTENT)
+        x = O.mesh_uniform(m + 1, L)
+        pt = O.target_position(x, O.TAU, bc == O.BC_DD)
+        wf = 0.4 * np.cos(1.7 * x) + 0.2
+        if bc == O.BC_DD:
+            cs = CubicSpline(x[1:-1], wf[1:-1], bc_type="natural")
+            wf[0], wf[-1] = float(cs(x[0])), float(cs(x[-1]))
+        else:
+            cs = CubicSpline(x[1:], wf[1:], bc_type="natural")
+            wf[0] = float(cs(x[0]))
+        n = len(unknowns(wf, bc))
+        w_nr = np.zeros(n + 1)
+        w_nr[1:] = unknowns(wf, bc)
+        out_nr = np.zeros(n + 1)
+        assert S.simple_FEM_1D_transient(w_nr, out_nr) == 0
+        ref = O.evaluate(x, wf, pt, N, bc=bc, cmat=O.C_CONSISTENT)
+        np.testing.assert_allclose(out_nr[1:], unknowns(ref["r"], bc), rtol=0, atol=PHI_ATOL)
+        S.finalize()
 
 
 def test_consistent_mass_with_wall_values():
