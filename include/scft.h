@@ -237,12 +237,15 @@ int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
 /* ---- Broyden with a batched Jacobian ------------------------------------
  * Replaces broydn(x, n, &check, simple_FEM_1D_transient) (1D_FEM.c:356,
  * broydn.c:44-292 with DEALII_SCFT/src/{fdjac,lnsrch,qrdcmp,qrupdt,rotate,
- * rsolv}.c, same constants) for one instance: F(w) = phi[w] - phi_t, the
+ * rsolv}.c; written from Dennis & Schnabel's secant method with the
+ * reference's constants) for one instance: F(w) = phi[w] - phi_t, the
  * n-column forward-difference Jacobian of fdjac.c:9-41 evaluated as ONE
  * batched sweep of n perturbed fields, the other evaluations on the device
  * one at a time, the n x n QR / rank-1 updates on the host (O(n^3) per
  * Jacobian: meant for n up to ~1000).  Instance as in scft_batch_desc (M
- * nodes at x, N steps, phi_t on the n unknowns, bc, cmat); w on the unknowns
+ * nodes at x, N steps, phi_t on the n unknowns, bc, cmat; with
+ * SCFT_C_CONSISTENT every field evaluated gets its wall values by natural-
+ * spline extrapolation, DEALII_SCFT/src/scft.cc:454-480); w on the unknowns
  * (0-based): in = start, out = result; tolf = the reference's `err`.
  * Returns SCFT_OK when broydn's check would be 0, SCFT_E_NOCONV where it
  * would be 1 (or where the reference would exit), SCFT_E_NAN / SCFT_E_PIVOT
@@ -257,6 +260,14 @@ typedef struct scft_broyden_stats {
 int scft_broyden_solve(int M, int N, const double *x, const double *phi_t,
                        int bc, int cmat, double tolf, double *w,
                        scft_broyden_stats *stats);
+/* The same iteration for a residual the caller computes: replaces
+ * broydn(x, n, &check, vecfunc) (broydn.c:44-46) with the callback
+ * F(n, w, f, ctx) on 0-based arrays, returning nonzero where the reference's
+ * vecfunc would set funcerr (broydn.c:37,82).  Each Jacobian is n further
+ * calls of F, one per column (fdjac.c:18-34).  Host only: no device work. */
+int scft_broyden_solve_fn(int n, int (*F)(int n, const double *w, double *f, void *ctx),
+                          void *ctx, double tolf, double *w,
+                          scft_broyden_stats *stats);
 
 /* ---- reference file formats ---------------------------------------------
  * .res files (Exp_m32_n2048_IE.res: the reader of 1D_FEM.c:321-342 skips

@@ -148,6 +148,8 @@ def _load():
                                    C.POINTER(C.c_int), C.c_void_p, C.c_void_p]
     L.scft_broyden_solve.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_int, C.c_int, C.c_double,
                                      _dp, C.POINTER(BroydenStats)]
+    L.scft_broyden_solve_fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, _dp,
+                                        C.POINTER(BroydenStats)]
     L.scft_resample_spline_batch.argtypes = [C.c_int, _ip, _lp, _dp, _dp, _ip, _lp, _dp, _dp]
     return L
 
@@ -229,6 +231,12 @@ class Batch:
         xoff[1:] = np.cumsum(self.M[:-1])
         self._x = _f64(np.concatenate(xs))
         self._pt = _f64(np.concatenate(phits))
+        if len(Ns) != len(xs) or len(phits) != len(xs):
+            raise ValueError(f"{len(xs)} meshes, {len(Ns)} step counts, {len(phits)} targets")
+        for k, p in enumerate(phits):
+            if len(p) != self.n[k]:
+                raise ValueError(f"instance {k}: target has {len(p)} values, the mesh "
+                                 f"{self.n[k]} unknowns")
         self._xoff = xoff
         d = BatchDesc(len(xs), self.M.ctypes.data_as(C.POINTER(C.c_int)),
                       self.N.ctypes.data_as(C.POINTER(C.c_int)),
@@ -259,10 +267,19 @@ class Batch:
         return [flat[o:o + n] for o, n in zip(self.uoff, self.n)]
 
     def join(self, parts):
-        """Concatenated unknowns of all instances (a flat array passes through)."""
+        """Concatenated unknowns of all instances (a flat array passes through).
+        Every part must hold exactly its instance's unknowns: the library
+        copies `total` doubles from the result."""
         if isinstance(parts, np.ndarray) and parts.ndim == 1:
-            assert parts.size == self.total
+            if parts.size != self.total:
+                raise ValueError(f"flat field has {parts.size} values, the batch "
+                                 f"{self.total} unknowns")
             return _f64(parts)
+        if len(parts) != len(self.n):
+            raise ValueError(f"{len(parts)} parts for {len(self.n)} instances")
+        for k, p in enumerate(parts):
+            if len(p) != self.n[k]:
+                raise ValueError(f"instance {k}: {len(p)} values for {self.n[k]} unknowns")
         return _f64(np.concatenate(parts))
 
     def eval(self, w_parts):
@@ -352,6 +369,9 @@ class Batch:
     def set_walls(self, wall0, wallL=None):
         """Per-instance field values at the walls for the consistent w-mass
         (default 0)."""
+        for v in (wall0, wallL):
+            if v is not None and np.ndim(v) > 0 and np.size(v) != len(self.M):
+                raise ValueError(f"{np.size(v)} wall values for {len(self.M)} instances")
         w0 = _f64(np.broadcast_to(np.asarray(wall0, dtype=np.float64), (len(self.M),)))
         wl = _f64(np.broadcast_to(np.asarray(0.0 if wallL is None else wallL, dtype=np.float64),
                                   (len(self.M),)))
@@ -487,6 +507,21 @@ def broyden_solve(x, phi_t_unknowns, N, w_unknowns, tolf=1e-8, bc=BC_DN, cmat=C_
     x, pt, w = _f64(x), _f64(phi_t_unknowns), _f64(w_unknowns).copy()
     st = BroydenStats()
     rc = lib.scft_broyden_solve(len(x), int(N), x, pt, bc, cmat, tolf, w, C.byref(st))
+    return dict(rc=rc, w=w, iterations=st.iterations, evaluations=st.evaluations,
+                sweeps=st.sweeps, final_err=st.final_err, check=st.check)
+
+
+BROYDEN_FN = C.CFUNCTYPE(C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                         C.c_void_p)
+
+
+def broyden_solve_fn(fn, w, tolf=1e-8, ctx=None):
+    """The same Broyden iteration on a residual the caller computes:
+    fn is a C function pointer (or a BROYDEN_FN) F(n, w, f, ctx) -> nonzero
+    on failure; the Jacobian columns are n further calls (host only)."""
+    w = _f64(w).copy()
+    st = BroydenStats()
+    rc = lib.scft_broyden_solve_fn(len(w), C.cast(fn, C.c_void_p), ctx, tolf, w, C.byref(st))
     return dict(rc=rc, w=w, iterations=st.iterations, evaluations=st.evaluations,
                 sweeps=st.sweeps, final_err=st.final_err, check=st.check)
 

@@ -1102,10 +1102,11 @@ static int dropin_eval(int n, const double* w, double* r) {
   for (int u = 0; u < n; u++) g_ptu[u] = g_ptall[u + 1];
   SCFT_TRY(batch_set_target(g_one, g_ptu.data()));
   // the consistent w-mass reads the field at the walls: extrapolated from the
-  // interior as the deal.II evaluator does (DD; needs 2+ interior nodes)
-  if (g_cfg.cmat == SCFT_C_CONSISTENT && dd && M >= 4) {
+  // unknowns by a natural spline as the deal.II evaluator does (DD: both
+  // walls; DN: the wall at x = 0; a single unknown keeps zero walls)
+  if (g_cfg.cmat == SCFT_C_CONSISTENT && n >= 2) {
     double w0 = 0.0, wl = 0.0;
-    if (!wall_values_natural(M, g_x.data(), w, &w0, &wl)) return SCFT_E_ARG;
+    if (!wall_values_from_unknowns(M, g_x.data(), w, dd, &w0, &wl)) return SCFT_E_ARG;
     SCFT_TRY(scft_batch_set_walls(g_one, &w0, &wl));
   }
   double rn = 0.0;

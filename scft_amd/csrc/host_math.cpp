@@ -177,6 +177,21 @@ bool wall_values_natural(int M, const double* x, const double* w_interior, doubl
   return true;
 }
 
+// The same for a field given on the unknowns of either boundary problem:
+// DD, nodes 1..M-2 (both walls extrapolated); DN, nodes 1..M-1 (x = 0
+// extrapolated from the natural spline through them, *wallL the field at
+// x = L itself, which is an unknown there).
+bool wall_values_from_unknowns(int M, const double* x, const double* w_unknowns, bool dd,
+                               double* wall0, double* wallL) {
+  if (dd) return wall_values_natural(M, x, w_unknowns, wall0, wallL);
+  if (M < 3) return false;
+  std::vector<double> Mi;
+  if (!spline_natural(M - 1, x + 1, w_unknowns, Mi)) return false;
+  *wall0 = spline_eval(M - 1, x + 1, w_unknowns, Mi.data(), x[0]);
+  *wallL = w_unknowns[M - 2];
+  return true;
+}
+
 // Value at xp of the spline (x, y, M): bisection for the bracketing
 // interval (the end intervals outside the table) and the cubic in the
 // form of spline_chen.c:88-100.

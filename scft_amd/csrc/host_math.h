@@ -13,6 +13,8 @@ double spline_eval(int n, const double* x, const double* y, const double* M, dou
 bool spline_natural(int n, const double* x, const double* y, std::vector<double>& M);
 bool wall_values_natural(int M, const double* x, const double* w_interior, double* wall0,
                          double* wallL);
+bool wall_values_from_unknowns(int M, const double* x, const double* w_unknowns, bool dd,
+                               double* wall0, double* wallL);
 std::vector<double> refine_by_gradient(int M, const double* x, const double* w_nodes,
                                        double factor, bool wall_right);
 bool free_energy(int M, const double* x, const double* w_nodes, double L, double tau, bool mirror,

@@ -47,7 +47,9 @@ constexpr unsigned ST_PIVOT = 4u;
 constexpr unsigned ST_MAXITER = 8u;
 constexpr unsigned ST_FALLBACK = 16u;
 constexpr unsigned ST_FAILED = 32u;
-constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER;
+// instances the march skips: converged, out of iterations, or given up by the
+// mixer (FAILED without a NaN: growth with no halvings left)
+constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER | ST_FAILED;
 
 struct MixState {
   int k;         // evaluations so far

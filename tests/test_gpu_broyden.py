@@ -81,3 +81,45 @@ def test_batched_jacobian_on_a_larger_instance():
     assert n_jac >= 1 and out["evaluations"] - out["sweeps"] == 255 * n_jac
     w = np.concatenate([[0.0], out["w"]])
     assert O.evaluate(x, w, pt, 1024, bc=O.BC_DN)["rnorm"] < 1e-8
+
+
+def test_consistent_mass_walls_follow_the_field():
+    """The consistent w-mass (deal.II, scft.cc:641-651) reads the field at the
+    walls; the deal.II evaluator extrapolates them from the interior by a
+    natural spline before every assembly (scft.cc:454-480), and so does
+    scft_broyden_solve for every field it evaluates, Jacobian columns
+    included.  Oracle: the same Broyden iteration (host-callback entry)
+    driving the CPU oracle evaluator with the walls extrapolated by scipy's
+    natural spline, from the deal.II fixture's field (DD, 33 nodes)."""
+    from scipy.interpolate import CubicSpline
+    S = _S()
+    fx = fixtures()["dealii_N33"]
+    x, w_full = np.array(fx["x"]), np.array(fx["w"])
+    N = 2048
+    pt = O.target_position(x, O.TAU, True)
+
+    def full(wu):
+        cs = CubicSpline(x[1:-1], wu, bc_type="natural")
+        return np.concatenate([[float(cs(x[0]))], wu, [float(cs(x[-1]))]])
+
+    def F(n, w, f, ctx):
+        wu = np.ctypeslib.as_array(w, shape=(n,)).copy()
+        r = O.evaluate(x, full(wu), pt, N, bc=O.BC_DD, cmat=O.C_CONSISTENT)["r"][1:-1]
+        for i in range(n):
+            f[i] = r[i]
+        return 0
+
+    cb = S.BROYDEN_FN(F)
+    ref = S.broyden_solve_fn(cb, w_full[1:-1], tolf=1e-8)
+    out = S.broyden_solve(x, pt[1:-1], N, w_full[1:-1], tolf=1e-8, bc=S.BC_DD,
+                          cmat=S.C_CONSISTENT)
+    print(f"device its {out['iterations']} evals {out['evaluations']}; oracle its "
+          f"{ref['iterations']} evals {ref['evaluations']}", flush=True)
+    assert out["rc"] == 0 and ref["rc"] == 0
+    assert np.abs(out["w"] - ref["w"]).max() <= 1e-6 * np.abs(ref["w"]).max()
+    r = O.evaluate(x, full(out["w"]), pt, N, bc=O.BC_DD, cmat=O.C_CONSISTENT)["rnorm"]
+    assert r < 1.01e-8
+    # with frozen zero walls the same start converges elsewhere: the walls matter
+    r0 = O.evaluate(x, np.concatenate([[0.0], out["w"], [0.0]]), pt, N, bc=O.BC_DD,
+                    cmat=O.C_CONSISTENT)["rnorm"]
+    assert r0 > 1e-7
