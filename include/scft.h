@@ -176,6 +176,11 @@ typedef struct scft_anderson_opts {
                              ||r|| exceeds backtrack * the last accepted one,
                              halves the last step and clears the history */
   int max_halvings;       /* consecutive halvings before the instance fails */
+  double field_floor;     /* in [0, 1): > 0 bounds every update below,
+                             w >= -field_floor / dt (dt = 1/N of the
+                             instance), so 1 + dt w >= 1 - field_floor and the
+                             implicit-Euler matrix stays strictly diagonally
+                             dominant (0: off, the reference's update) */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

@@ -67,6 +67,7 @@ int orc_solve_anderson(const orc_problem *p, double *w_full,
   int n = orc_n_unknowns(p->M, p->bc);
   memcpy(c.wfull, w_full, sizeof(double) * p->M);
   orc_anderson_opts oo = *o;
+  if (oo.field_floor > 0.0) oo.field_lower = -oo.field_floor * p->N;
   double *ps = NULL;
   if (oo.precond > 0.0 && !oo.pstencil) { /* stencil of this instance's mesh */
     ps = (double *)malloc(sizeof(double) * 3 * (size_t)n);

@@ -286,6 +286,8 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
     }
     double scale = (o->step_cap > 0.0 && step > o->step_cap) ? o->step_cap / step : 1.0;
     for (int i = 0; i < n; i++) w[i] += scale * (wn[i] - w[i]);
+    if (o->field_floor > 0.0)
+      for (int i = 0; i < n; i++) w[i] = fmax(w[i], o->field_lower);
   }
   if (st) {
     st->iterations = k + (rc == 0 ? 1 : 0);

@@ -39,6 +39,12 @@ typedef struct orc_anderson_opts {
                            exceeds backtrack * (last accepted error) halves
                            the last step and clears the history */
   int max_halvings;     /* consecutive halvings allowed before giving up */
+  double field_floor;   /* > 0: after each update w >= -field_floor / dt,
+                           so 1 + dt w >= 1 - field_floor and the implicit-
+                           Euler matrix stays strictly diagonally dominant */
+  double field_lower;   /* the bound itself, -field_floor * N: set from the
+                           instance's N by orc_solve_anderson (a caller of
+                           orc_anderson sets it directly) */
 } orc_anderson_opts;
 
 /* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements

@@ -63,7 +63,8 @@ class AndersonOpts(C.Structure):
                 ("transform", C.c_int), ("floor", C.c_double),
                 ("fallback_factor", C.c_double), ("check_every", C.c_int),
                 ("progress_seconds", C.c_double), ("precond", C.c_double),
-                ("backtrack", C.c_double), ("max_halvings", C.c_int)]
+                ("backtrack", C.c_double), ("max_halvings", C.c_int),
+                ("field_floor", C.c_double)]
 
 
 class BroydenStats(C.Structure):
@@ -208,10 +209,10 @@ def local_iteration() -> int:
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
                   fallback_factor=0.0, check_every=8, progress_seconds=0.0, precond=0.0,
-                  backtrack=0.0, max_halvings=30):
+                  backtrack=0.0, max_halvings=30, field_floor=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, fallback_factor, check_every, progress_seconds,
-                        precond, backtrack, max_halvings)
+                        precond, backtrack, max_halvings, field_floor)
 
 
 class Batch:

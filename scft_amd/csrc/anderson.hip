@@ -22,7 +22,14 @@
 //    so I + K/2 undoes the response at short wavelengths;
 //  * backtracking: a failed evaluation (NaN / zero pivot) or one whose ||r||
 //    exceeds backtrack times the last accepted one halves the step toward the
-//    last accepted iterate (ring slot head - 1) and clears the history.
+//    last accepted iterate (ring slot head - 1) and clears the history;
+//  * a lower bound w >= -field_floor / dt on the updated field.  Near a wall
+//    on a fine mesh the converged field spans decades (w at the first node
+//    1e6-1e8 when dt/h^2 ~ 1e4-1e5), and the preconditioned step from a smooth
+//    start overshoots to large negative values at the next nodes, where
+//    1 + dt w < 0 makes the implicit-Euler matrix indefinite and the density
+//    explode; with 1 + dt w >= 1 - field_floor every row stays strictly
+//    diagonally dominant.
 #include "scft_internal.h"
 
 namespace scft {
@@ -338,6 +345,9 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   }
 
   // ----------------------------------------------------------- update
+  // lower bound of the field (field_floor > 0): 1 + dt w >= 1 - field_floor
+  // keeps every row of the implicit-Euler matrix strictly diagonally dominant
+  const double lo = a.field_floor > 0.0 ? -a.field_floor * d.N : -INFINITY;
   double stepmax = 0.0;
   for (int u = tid; u < n; u += AM_THREADS) {
     const double wk = w[u];
@@ -351,7 +361,7 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     }
     const double wn = wb + a.beta * rb;
     stepmax = fmax(stepmax, fabs(wn - wk));
-    w[u] = wn;  // provisional, rescaled below
+    w[u] = a.step_cap > 0.0 ? wn : fmax(wn, lo);  // capped: rescaled and bounded below
   }
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) stepmax = fmax(stepmax, __shfl_xor(stepmax, off));
@@ -364,10 +374,10 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   }
   __syncthreads();
   const double scale = s_scale;
-  if (scale != 1.0) {
+  if (scale != 1.0 || (a.step_cap > 0.0 && a.field_floor > 0.0)) {
     for (int u = tid; u < n; u += AM_THREADS) {
       const double wk = hw[(long long)cur * n + u];
-      w[u] = wk + scale * (w[u] - wk);
+      w[u] = fmax(wk + scale * (w[u] - wk), lo);
     }
   }
   if (tid == 0) {

@@ -70,17 +70,11 @@ struct Bucket {
   int stream = 0;             // which of the batch's streams launches it
 };
 
-// rows per lane of a new bucket: implicit Euler keeps ~8 doubles per row in
-// registers, IRK4 ~11 (its variants stop at K = 8)
-int default_K(int cap, int stepper) {
-  return std::min(cap / 64, stepper == SCFT_STEP_IRK4 ? 8 : 16);
-}
-
-int cap_for(int n) {
-  int c = 64;
-  while (c < n) c *= 2;
-  return c;
-}
+// rows per lane of a new bucket (implicit Euler keeps ~8 doubles per row in
+// registers, IRK4 ~11: its variants stop at K = 8) and the bucket of n
+// unknowns: shared with the allocation-free entry point (scft_internal.h)
+int default_K(int cap, int stepper) { return bucket_default_K(cap, stepper == SCFT_STEP_IRK4); }
+int cap_for(int n) { return bucket_capacity(n); }
 
 }  // namespace
 
@@ -759,6 +753,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->precond = 0.0;
   o->backtrack = 0.0;
   o->max_halvings = 30;
+  o->field_floor = 0.0;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -813,6 +808,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   if (!b || !o) return SCFT_E_ARG;
   if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
   if (!(o->precond >= 0.0) || !(o->backtrack >= 0.0) || o->max_halvings < 0) return SCFT_E_ARG;
+  if (!(o->field_floor >= 0.0 && o->field_floor < 1.0)) return SCFT_E_ARG;
   SCFT_TRY(ensure_mixer(b, o->history + 1));
   const auto t0 = std::chrono::steady_clock::now();
   const int nb = static_cast<int>(b->buckets.size());
@@ -846,6 +842,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.precond = o->precond;
   a.backtrack = o->backtrack;
   a.max_halvings = o->max_halvings;
+  a.field_floor = o->field_floor;
   // fresh mixer state (a small kernel-free host copy of zeros is not needed:
   // the per-instance state array is rewritten from a pinned staging copy)
   if (static_cast<int>(b->ms0.size()) != b->n_inst)

@@ -48,12 +48,37 @@ bool romberg_weights(int m, double hh, std::vector<double>& c) {
 // are symmetric (c_j = c_{N-j}), so phi = c_{N/2} q_{N/2}^2 +
 // sum_{j > N/2} 2 c_j q_j q_{N-j}; pw[t] belongs to step N/2 + t.
 bool pairing_weights(int N, std::vector<double>& pw) {
-  std::vector<double> c;
-  if (!romberg_weights(N, 1.0 / N, c)) return false;
+  if (N < 16 || (N & (N - 1)) != 0) return false;
+  pw.assign(N / 2 + 1, 0.0);
+  return pairing_weights_into(N, pw.data());
+}
+
+// The same into a caller's array of N/2 + 1 doubles, with no allocation:
+// each c_j summed over the five levels in the order romberg_weights uses, so
+// the weights are the same bits.
+bool pairing_weights_into(int N, double* pw) {
+  const int K = 5;
+  if (N < 16 || (N & (N - 1)) != 0) return false;
+  const double hh = 1.0 / N;
+  const int levels = static_cast<int>(std::log(N * 1.0) / 0.6931471805599453 + 1.5);
+  double h[K], e[K], lam[K];
+  for (int k = 0; k < K; k++) h[k] = std::pow(0.25, levels - K + k);
+  for (int k = 0; k < K; k++) {
+    for (int t = 0; t < K; t++) e[t] = (t == k) ? 1.0 : 0.0;
+    lam[k] = neville_zero(h, e, K);
+  }
   const int half = N / 2;
-  pw.assign(half + 1, 0.0);
-  pw[0] = c[half];
-  for (int t = 1; t <= half; t++) pw[t] = 2.0 * c[half + t];
+  for (int t = 0; t <= half; t++) {
+    const int j = half + t;
+    double c = 0.0;
+    for (int k = 0; k < K; k++) {
+      const int stride = N >> (levels - K + k);
+      if (j % stride) continue;
+      const double panel = stride * hh;
+      c += lam[k] * ((j == 0 || j == N) ? 0.5 * panel : panel);
+    }
+    pw[t] = t == 0 ? c : 2.0 * c;
+  }
   return true;
 }
 

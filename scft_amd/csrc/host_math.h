@@ -5,6 +5,7 @@
 namespace scft {
 bool romberg_weights(int m, double hh, std::vector<double>& c);
 bool pairing_weights(int N, std::vector<double>& pw);
+bool pairing_weights_into(int N, double* pw);
 double exp_profile(double x, double tau);
 void target_position(int M, const double* x, double tau, bool mirror, double* phi_t);
 void mesh_uniform(int M, double L, double* x);

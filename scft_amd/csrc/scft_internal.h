@@ -86,7 +86,22 @@ struct AndersonArgs {
   double precond;          // Laplacian preconditioner coefficient (0: off)
   double backtrack;        // growth factor that triggers a halving (0: off)
   int max_halvings;
+  double field_floor;      // w >= -field_floor * N after each update (0: off)
 };
+
+// Size bucket of an instance with n unknowns: the power of two >= n (at
+// least 64) = 64 * W * K lanes-times-rows of the workgroup that marches it.
+inline int bucket_capacity(int n) {
+  int c = 64;
+  while (c < n) c *= 2;
+  return c;
+}
+// Rows per lane of a bucket before any tuning: implicit Euler keeps ~8
+// doubles per row in registers, IRK4 ~11 (its variants stop at K = 8).
+inline int bucket_default_K(int cap, bool irk4) {
+  const int k = cap / 64, kmax = irk4 ? 8 : 16;
+  return k < kmax ? k : kmax;
+}
 
 // Launch one bucket.  Returns hipSuccess or the launch error.
 hipError_t launch_march(int K, int W, const MarchArgs& a, int grid,

@@ -7,11 +7,12 @@ h(x) = h0 (1 + x/ell), h0 log-uniform in [1e-4, 1e-2], ell in [0.05, 0.5],
   oracle's own FP64 march agrees with its extended-precision march to ~1e-12
   on the stiffest of these shapes, dt/h0^2 ~ 2e5).
 * A batched solve of a 1024-instance sample with the preconditioned log
-  mixer.  The stiffest instances (dt/h0^2 >~ 1e4, where the converged field
-  at the first node reaches 1e6-1e8) do not all converge with this mixer: in
-  this sample about 90% do (every failure has dt/h0^2 >= 1.4e4).  The test
-  prints the fraction, requires at least 85%, and checks converged fields
-  against the oracle's evaluator.
+  mixer and the field bound w >= -0.5 / dt.  Without the bound about 10% of
+  the sample (every instance with dt/h0^2 >~ 1.4e4, where the converged field
+  at the first node reaches 1e6-1e8) never converges: the first
+  preconditioned steps drive the next nodes' field to large negative values,
+  1 + dt w < 0, and the density explodes.  With it every instance converges;
+  converged fields are checked against the oracle's evaluator.
 """
 import numpy as np
 import pytest
@@ -65,7 +66,7 @@ def test_config5_sample_solve():
     b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
     opts = S.anderson_opts(history=16, beta=1.0, tol=1e-6, max_iter=200, step_cap=0.0,
                            transform=S.MIX_LOG, floor=1e-8, precond=0.5, backtrack=1e30,
-                           check_every=4)
+                           check_every=4, field_floor=0.5)
     out = b.solve([cfg["w0"][k][1:] for k in idx], opts)
     b.close()
     conv = (out["status"] & S.ST_CONVERGED) != 0
@@ -74,9 +75,11 @@ def test_config5_sample_solve():
     print(f"config5 sample: {conv.sum()}/{len(idx)} converged, mean iters "
           f"{out['iters'][conv].mean():.1f}, max {out['iters'][conv].max()}; unconverged "
           f"dt/h0^2 min {stiff[~conv].min() if (~conv).any() else 0:.2e}", flush=True)
-    assert conv.mean() >= 0.85
-    # the converged fields solve the oracle's evaluator to the tolerance
-    for j in np.flatnonzero(conv)[:: max(1, conv.sum() // 16)][:16]:
+    assert conv.all(), np.flatnonzero(~conv)
+    # the converged fields solve the oracle's evaluator to the tolerance: the
+    # 8 stiffest instances and 8 others
+    audit = list(np.argsort(-stiff)[:8]) + list(rng.choice(len(idx), 8, replace=False))
+    for j in audit:
         wf = np.concatenate([[0.0], out["w"][j]])
         r = O.evaluate(xs[j], wf, pts[j], Ns[j], bc=O.BC_DN)["rnorm"]
         assert r <= 1.05e-6, (j, r)

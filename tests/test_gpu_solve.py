@@ -76,6 +76,39 @@ def test_production_mixer_trajectory_twelve_updates():
     b.close()
 
 
+def test_field_floor_trajectory_matches_oracle():
+    """The lower bound w >= -field_floor / dt on stiff graded instances of the
+    production mix (dt/h0^2 ~ 1.6e4 and 2.8e4), where the first preconditioned
+    steps push dozens of wall-side nodes onto the bound: device and oracle
+    agree over 6 updates (bar 1e-8 relative; two oracle runs 1e-13 apart in
+    the start stay within 1e-19 there)."""
+    S = _S()
+    cfg = WL.config5(n_inst=2048)
+    pick = [1658, 1640]
+    xs = [cfg["xs"][k] for k in pick]
+    Ns = [cfg["Ns"][k] for k in pick]
+    pts = [cfg["pts"][k] for k in pick]
+    w0 = [cfg["w0"][k].copy() for k in pick]
+    for w in w0:
+        w[0] = 0.0
+    for K in (2, 6):
+        o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0,
+                                  max_iter=K + 1, precond=0.5, field_floor=0.5)
+        o_cpu.max_iter = K
+        b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
+        out = b.solve([w[1:] for w in w0], o_dev)
+        b.close()
+        for j in range(len(pick)):
+            ref = O.solve_anderson(xs[j], pts[j], Ns[j], w0[j], o_cpu, bc=O.BC_DN)
+            if K == 2:  # the bound is active
+                assert ref["w"][1:].min() == -0.5 * Ns[j]
+                assert out["w"][j].min() == -0.5 * Ns[j]
+            dw = np.abs(out["w"][j] - ref["w"][1:]).max() / max(1.0, np.abs(ref["w"]).max())
+            print(f"instance {pick[j]} M={len(xs[j])} N={Ns[j]}: {K} updates, relative "
+                  f"|w_dev - w_cpu| = {dw:.2e}", flush=True)
+            assert dw <= 1e-8
+
+
 def test_device_anderson_converges_config1():
     """Config 1 (m = 32, N = 2048, w0 = 0), history 10, mixing 0.1, cap 0.5:
     converges on the device and the converged field evaluates (on the CPU
