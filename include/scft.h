@@ -239,6 +239,30 @@ int scft_batch_tune_placement(scft_batch *b, int tries, int reps);
 int scft_batch_tuning(const scft_batch *b, int *n_buckets, int *bucket_nmax,
                       int *K, int *W, int max_buckets);
 
+/* ---- evaluation without allocation ----------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): scft_batch_eval of ONE instance with inputs and outputs
+ * in the caller's DEVICE memory -- d_x (M node coordinates), d_w and d_phi_t
+ * (n unknowns) in; d_phi (n, nullable), d_r = phi - phi_t (n), d_rnorm
+ * (1 double) and d_status (1 word, SCFT_ST_* bits) out.  Walls at w = 0, the
+ * implicit-Euler stepper, the march variant of a one-instance batch (so the
+ * same bits as scft_batch_eval).  Calls no allocating routine: the working
+ * memory (descriptor, pairing weights, work counter, q-slabs) is a fixed
+ * device array of 64 MiB + 256 KiB per GPU that the library holds until the
+ * process ends.  The slabs need (N/2) * cap * 8 bytes, cap = the power of
+ * two >= max(n, 64): up to n = 4096 with N = 4096, n = 2048 with N = 8192,
+ * n = 1024 with N = 16384; larger instances return SCFT_E_ARG (message in
+ * scft_noalloc_error()).  Every call on a GPU uses that GPU's one array, so
+ * the library serialises calls within a process and calls that overlap on
+ * one GPU from elsewhere would race.  `stream`: a hipStream_t (NULL: the
+ * null stream); returns after the work on it has finished. */
+int scft_eval_noalloc(int M, int N, int bc, int cmat, const double *d_x,
+                      const double *d_w, const double *d_phi_t, double *d_phi,
+                      double *d_r, double *d_rnorm, unsigned *d_status,
+                      void *stream);
+const char *scft_noalloc_error(void);
+
 /* ---- Broyden with a batched Jacobian ------------------------------------
  * Replaces broydn(x, n, &check, simple_FEM_1D_transient) (1D_FEM.c:356,
  * broydn.c:44-292 with DEALII_SCFT/src/{fdjac,lnsrch,qrdcmp,qrupdt,rotate,

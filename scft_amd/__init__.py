@@ -151,6 +151,8 @@ def _load():
                                      _dp, C.POINTER(BroydenStats)]
     L.scft_broyden_solve_fn.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_double, _dp,
                                         C.POINTER(BroydenStats)]
+    L.scft_eval_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int] + [C.c_void_p] * 8
+    L.scft_noalloc_error.restype = C.c_char_p
     L.scft_resample_spline_batch.argtypes = [C.c_int, _ip, _lp, _dp, _dp, _ip, _lp, _dp, _dp]
     return L
 
@@ -525,6 +527,20 @@ def broyden_solve_fn(fn, w, tolf=1e-8, ctx=None):
     rc = lib.scft_broyden_solve_fn(len(w), C.cast(fn, C.c_void_p), ctx, tolf, w, C.byref(st))
     return dict(rc=rc, w=w, iterations=st.iterations, evaluations=st.evaluations,
                 sweeps=st.sweeps, final_err=st.final_err, check=st.check)
+
+
+# ---------------------------------------------------------------- no allocation
+def eval_noalloc(M, N, d_x, d_w, d_phi_t, d_phi, d_r, d_rnorm, d_status, bc=BC_DN,
+                 cmat=C_ROWSCALED, stream=None):
+    """scft_eval_noalloc on caller-owned device buffers (integer addresses);
+    returns the status code (SCFT_E_ARG with scft_noalloc_error() for an
+    instance beyond the fixed scratch)."""
+    return lib.scft_eval_noalloc(int(M), int(N), bc, cmat, d_x, d_w, d_phi_t, d_phi, d_r,
+                                 d_rnorm, d_status, stream)
+
+
+def noalloc_error() -> str:
+    return (lib.scft_noalloc_error() or b"").decode()
 
 
 # ---------------------------------------------------------------- file formats
