@@ -18,7 +18,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# SCFT_ORACLE_LIB: another build of the same sources (the sanitizer build of
+# the CPU test suite loads oracle/_san/liboracle_san.so this way)
+LIB_PATH = os.environ.get("SCFT_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 REF_DIR = os.path.join(HERE, "_ref")
 REFERENCE_ROOT = "/root/reference"
 
