@@ -1,35 +1,45 @@
 """Benchmark: batched SCFT solves/s to ||phi - phi_t||_inf <= 1e-6 on one GPU.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n-inst 4096]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload config5|config3]
+                    [--n-inst N]
 
-Workload (configuration 3): 4096 independent instances, mesh sizes m
-log-uniform in [32, 4096], contour steps N uniform over {512, 1024, 2048,
-4096}, L uniform in [2, 8], w0 = 0.1 exp(-x/0.2) + N(0, 1e-3) per instance
-seed; Dirichlet wall at x = 0, zero-flux end at L, EXP wall target.
+Workloads (independent instances, Dirichlet wall at x = 0, zero-flux end at
+L, EXP wall target, w0 = 0.1 exp(-x/0.2) + N(0, 1e-3) per instance seed):
 
-A step is one complete batched solve of the 4096 instances with the fields
-already resident in HBM: every instance is driven by the device Anderson
-mixer until its residual is below 1e-6.  Two legs are timed, K steps each:
+  config5 (default)  the production mix: 16384 instances on meshes graded
+                     toward the wall, h(x) = h0 (1 + x/ell), h0 log-uniform
+                     in [1e-4, 1e-2], ell uniform in [0.05, 0.5], 64-4096
+                     nodes, N uniform over {512, ..., 8192}, L in [2, 8]
+  config3            4096 uniform meshes, m log-uniform in [32, 4096], N in
+                     {512, 1024, 2048, 4096}, L in [2, 8]
+
+A step is one complete batched solve of every instance with the fields
+already resident in HBM: the device Anderson mixer drives each instance
+until its residual is below 1e-6.  Two legs are timed, K steps each:
 
   cold   every instance starts from w0 (the reference's initial guess)
   warm   every instance starts from the prediction of the warm-start MLP,
-         trained (untimed setup) on 20000 fields that the batched solver
-         itself converged for instances drawn like the bench batch from
-         disjoint seeds; the timed step includes the prediction (network
-         forward on the device, spline resampling onto every mesh); an
-         instance whose first residual exceeds 10x the batch median
-         restarts from w0
+         trained (untimed setup) on fields that the batched solver itself
+         converged for instances of the same workload drawn from disjoint
+         seeds; the timed step includes the prediction (network forward on
+         the device, spline resampling onto every mesh); an instance whose
+         first residual exceeds 10x the batch median restarts from w0
 
 `value` = warm-start instances solved per second (cold with --no-warm);
 config.cold_start holds the cold leg and config.warm_over_cold the ratio.
 
 Also reported:
+  audit         64 random instances of the warm solve: the device's density
+                of its converged field against the CPU oracle's evaluation of
+                the same field, and the oracle's residual of it
   roofline      the dominant march launch (largest per-bucket time) of one
                 evaluation sweep: algorithmic bytes (q-slabs written and read
                 back, O(n) I/O) over its HIP-event duration, against 8 TB/s
   cpu_baseline  the CPU oracle (same mixer, same tolerance, cold start) on a
                 bounded sample of the same instances, threads over the host
-                cores
+                cores the GPU box allots to one GPU; beside it the reference
+                problem of configuration 1 (m = 32, N = 2048, w0 = 0, plain
+                Anderson, history 10, mixing 0.1) solved single-threaded
 """
 from __future__ import annotations
 
@@ -53,14 +63,46 @@ def log(msg):
 
 
 # The mixer of both sides: history 16 (the fine meshes' wall layer has many
-# slowly converging modes: history 10 needs up to 32 evaluations on this batch,
-# 16 needs 25), log(phi/phi_t) mixing residual preconditioned by
-# I + 0.5 M_L^{-1} K, beta 1, no step cap, a NaN evaluation halves the step
-# (backtrack factor 1e30: NaN only).
-SOLVER = dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8, precond=0.5,
-              backtrack=1e30, max_halvings=30)
-SOLVER_TEXT = ("device Anderson, history 16, log mixing preconditioned by I + 0.5 M^-1 K, "
-               "beta 1.0, no step cap, NaN backtracking")
+# slowly converging modes: history 10 needs up to 32 evaluations on the
+# uniform-mesh batch, 16 needs 25), log(phi/phi_t) mixing residual
+# preconditioned by I + a M_L^{-1} K, beta 1, no step cap, a NaN evaluation
+# halves the step (backtrack factor 1e30: NaN only), and the field bounded
+# below by -0.5/dt (1 + dt w >= 1/2: the implicit-Euler matrix stays
+# diagonally dominant; without it ~10% of the graded meshes never converge).
+# a = 0.5 on the uniform meshes (their finest 48: 20.1 mean / 21 max
+# evaluations against 20.8 / 22 at 0.6); a = 0.6 on the graded ones (their
+# slowest 8: 30 / 36 against 83 / 101 at 0.5; 40 stiff ones 30 / 41 against
+# 34 / 58).
+WORKLOADS = {
+    "config5": dict(
+        n_inst=16384, seed=20261205, precond=0.6,
+        text="config5: {n} DN instances on graded meshes h(x) = h0 (1 + x/ell), h0 "
+             "log-uniform [1e-4,1e-2], ell in [0.05,0.5], 64-4096 nodes, N in "
+             "{{512,...,8192}}, L in [2,8], tol 1e-6"),
+    "config3": dict(
+        n_inst=4096, seed=20261202, precond=0.5,
+        text="config3: {n} DN instances, m log-uniform [32,4096], N in {{512,1024,2048,4096}}, "
+             "L in [2,8], tol 1e-6"),
+}
+
+
+def solver_settings(workload):
+    return dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8,
+                precond=WORKLOADS[workload]["precond"], backtrack=1e30, max_halvings=30,
+                field_floor=0.5)
+
+
+def solver_text(workload):
+    return (f"device Anderson, history 16, log mixing preconditioned by I + "
+            f"{WORKLOADS[workload]['precond']} M^-1 K, beta 1.0, no step cap, NaN "
+            "backtracking, field bounded below by -0.5/dt")
+
+
+def make_workload(WL, workload, n_inst, seed=None):
+    seed = WORKLOADS[workload]["seed"] if seed is None else seed
+    if workload == "config5":
+        return WL.config5(n_inst=n_inst, seed=seed)
+    return WL.config3(n_inst=n_inst, seed=seed)
 
 
 def march_source_sha256():
@@ -74,12 +116,12 @@ def march_source_sha256():
     return h.hexdigest()
 
 
-def measured_traffic(kern, n_items, alg_bytes):
+def measured_traffic(workload, kern, n_items, alg_bytes):
     """HBM bytes of the dominant launch from the committed counter runs
     (FETCH_SIZE + WRITE_SIZE, copy-calibrated), when they were taken on this
     same launch (kernel source, variant, bucket, algorithmic bytes);
     otherwise None."""
-    path = os.path.join(ROOT, "profiles", "march_config3_4096_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"march_{workload}_traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
@@ -97,17 +139,19 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     for `args.train_inst` instances drawn like the bench batch from disjoint
     seeds (setup, untimed), then time K warm-started solves of the bench
     batch.  A timed step covers the prediction (network forward on the
-    device, spline resampling onto every mesh) and the device-resident solve;
+    device, spline resampling onto every mesh, the upload of the predicted
+    fields) and the device-resident solve;
     an instance whose first residual exceeds 10x the batch median restarts
     from w0."""
     from scft_amd import warmstart as WS
     t0 = time.perf_counter()
     Xs, Ys = [], []
-    opts = solver_opts(S, args.max_iter)
+    opts = solver_opts(S, args)
     left, part = args.train_inst, 0
     while left > 0:
-        n = min(4096, left)
-        tcfg = WL.config3(n_inst=n, seed=7000 + 101 * part)  # seeds disjoint from the bench
+        n = min(8192, left)
+        # seeds disjoint from the bench batch's (parameters and noise)
+        tcfg = make_workload(WL, args.workload, n, seed=7000 + 101 * part)
         out = WS.solve(S, tcfg, opts)
         conv = (out["status"] & S.ST_CONVERGED) != 0
         X, Y = WS.dataset(S, tcfg, out["w_nodes"], conv)
@@ -124,15 +168,15 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     log(f"bench: warm start: {len(X)} training fields in {t_data:.1f} s, 50 epochs in "
         f"{t_train:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
     prep = WS.Prepared(S, cfg)
-    wopts = solver_opts(S, args.max_iter)
+    wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
     w0_flat = b.join(w0)
 
     def step():
         t = time.perf_counter()
         wi = ws.predict_unknowns(prep)
+        b.upload(wi, w0_flat)  # the predicted fields (and the fallback's) into HBM
         t_pred = time.perf_counter() - t
-        b.upload(wi, w0_flat)
         r = b.solve_resident(wopts)
         return t_pred, r
 
@@ -151,7 +195,8 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
             f"{float(np.mean(res['iters'])):.1f}, max {int(np.max(res['iters']))}, {fb} fallbacks")
     ws.close()
     info = {
-        "network": "131 -> 4 x 256 tanh -> 136, FP32, device forward/backward/Adam kernels",
+        "network": f"{WS.DIMS[0]} -> 4 x 256 tanh -> {WS.DIMS[-1]}, FP32, device "
+                   "forward/backward/Adam kernels",
         "training_fields": int(len(X)), "epochs": 50, "batch": 256, "lr": 1e-3,
         "final_loss": round(float(hist[-1]), 6), "train_seconds": round(t_train, 2),
         "predict_ms_per_step": round(1e3 * tp / args.steps, 2),
@@ -160,9 +205,9 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     return dict(res=res, total=total, value=res["converged"] * args.steps / total, info=info)
 
 
-def solver_opts(S, max_iter):
-    return S.anderson_opts(max_iter=max_iter, transform=S.MIX_LOG, check_every=4,
-                           progress_seconds=20.0, **SOLVER)
+def solver_opts(S, args):
+    return S.anderson_opts(max_iter=args.max_iter, transform=S.MIX_LOG, check_every=4,
+                           progress_seconds=20.0, **solver_settings(args.workload))
 
 
 def stepper_compare(args):
@@ -192,7 +237,7 @@ def stepper_compare(args):
             f"{n} instances, ||r|| of the reference field {out['rnorm'][0]:.2e}")
 
 
-def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
+def cpu_baseline(cfg, gpu_iters, args, threads, budget_s):
     """The CPU oracle (same mixer, options and tolerance) solving a random
     sample of the batch on `threads` OpenMP threads.  The sample is drawn
     until its estimated CPU work reaches `budget_s`; the full-batch CPU time
@@ -214,7 +259,8 @@ def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
             break
         idx.append(int(k))
         est += work[k] * c
-    opts = O.anderson_opts(max_iter=max_iter, transform=O.MIX_LOG, **SOLVER)
+    opts = O.anderson_opts(max_iter=args.max_iter, transform=O.MIX_LOG,
+                           **solver_settings(args.workload))
     t0 = time.perf_counter()
     out = O.batch_solve([len(cfg["xs"][k]) for k in idx], [cfg["Ns"][k] for k in idx],
                         [cfg["xs"][k] for k in idx], [cfg["pts"][k] for k in idx],
@@ -228,12 +274,65 @@ def cpu_baseline(cfg, gpu_iters, max_iter, threads, budget_s):
                 full_time=full_time)
 
 
+def config1_single_thread(S, WL):
+    """The reference problem of configuration 1 (m = 32, N = 2048, w0 = 0,
+    DN, L = 3.72) with the reference's plain Anderson mixing (history 10,
+    mixing 0.1, step cap 0.5) to 1e-6: the CPU oracle on one thread, as the
+    reference runs (1D_FEM.c:208-228), beside the device solving the same
+    single instance."""
+    import oracle as O
+    cfg = WL.config1()
+    x, N, pt, w0 = cfg["xs"][0], cfg["Ns"][0], cfg["pts"][0], cfg["w0"][0]
+    opts = O.anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5)
+    reps, t0 = 0, time.perf_counter()
+    while reps < 3 or time.perf_counter() - t0 < 2.0:
+        cpu = O.solve_anderson(x, pt, N, w0, opts, bc=O.BC_DN)
+        reps += 1
+    t_cpu = (time.perf_counter() - t0) / reps
+    b = S.Batch([x], [N], [pt[1:]], bc=S.BC_DN)
+    dopts = S.anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
+                            check_every=8)
+    b.solve([w0[1:]], dopts)  # warm-up (graph instantiation)
+    t0 = time.perf_counter()
+    dev = b.solve([w0[1:]], dopts)
+    t_dev = time.perf_counter() - t0
+    b.close()
+    return {"workload": "config1: m=32, N=2048, L=3.72, w0=0, plain Anderson history 10, "
+                        "mixing 0.1, tol 1e-6",
+            "cpu_one_thread_seconds": round(t_cpu, 4),
+            "cpu_evaluations": int(cpu["iterations"]),
+            "cpu_converged": bool(cpu["rc"] == 0),
+            "gpu_seconds": round(t_dev, 4),
+            "gpu_evaluations": int(dev["iters"][0]),
+            "gpu_converged": bool(dev["status"][0] & S.ST_CONVERGED)}
+
+
+def audit(S, cfg, b, res, n_audit=64, threads=16):
+    """Audit subset of a solve: the device's density of each converged field
+    against the oracle's evaluation of the same field, and the oracle's
+    residual of it."""
+    import oracle as O
+    out = b.eval(res["w"])
+    rng = np.random.default_rng(64)
+    idx = rng.choice(len(cfg["xs"]), n_audit, replace=False)
+    wf = [np.concatenate([[0.0], res["w"][k]]) for k in idx]
+    ref = O.batch_eval([len(cfg["xs"][k]) for k in idx], [cfg["Ns"][k] for k in idx],
+                       [cfg["xs"][k] for k in idx], [cfg["pts"][k] for k in idx], wf,
+                       bc=O.BC_DN, threads=threads)
+    dphi = max(float(np.abs(out["phi"][k] - ref["phi"][j][1:]).max()) for j, k in enumerate(idx))
+    return {"instances": int(n_audit), "max_abs_phi_dev_minus_oracle": dphi,
+            "max_oracle_residual": float(np.max(ref["rnorm"])),
+            "converged": int(np.sum((res["status"][idx] & S.ST_CONVERGED) != 0))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--n-inst", type=int, default=4096)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="config5")
+    ap.add_argument("--n-inst", type=int, default=0,
+                    help="instances (default: the workload's, 16384 / 4096)")
     ap.add_argument("--max-iter", type=int, default=500)
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of CPU-baseline work (per thread) to time")
@@ -276,8 +375,10 @@ def main():
     import scft_amd as S
     from scft_amd import workloads as WL
 
-    log(f"bench: generating configuration 3 ({args.n_inst} instances)")
-    cfg = WL.config3(n_inst=args.n_inst)
+    if args.n_inst <= 0:
+        args.n_inst = WORKLOADS[args.workload]["n_inst"]
+    log(f"bench: generating {args.workload} ({args.n_inst} instances)")
+    cfg = make_workload(WL, args.workload, args.n_inst)
     w0 = [w[1:] for w in cfg["w0"]]
     b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN,
                 tune_file=args.tune_file)
@@ -308,7 +409,7 @@ def main():
         return
 
     # ---- cold start: every instance from w0 (the reference's initial guess)
-    opts = solver_opts(S, args.max_iter)
+    opts = solver_opts(S, args)
     for i in range(args.warmup):
         b.upload(w0)
         r = b.solve_resident(opts)
@@ -348,11 +449,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"config3: {args.n_inst} DN instances, m log-uniform [32,4096], "
-                        "N in {512,1024,2048,4096}, L in [2,8], tol 1e-6",
+            "workload": WORKLOADS[args.workload]["text"].format(n=args.n_inst),
             "n_inst": args.n_inst,
             "start": "warm (MLP) with cold fallback" if warm else "cold",
-            "solver": SOLVER_TEXT,
+            "solver": solver_text(args.workload),
             "converged": int(res["converged"]),
             "mean_iters": round(float(np.mean(res["iters"])), 2),
             "max_iters": int(np.max(res["iters"])),
@@ -371,16 +471,23 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(kern, n_items, by),
+            "traffic": measured_traffic(args.workload, kern, n_items, by),
             "kernel": f"{kern}: bucket of {n_items} instances, {ms:.3f} ms per launch",
         },
     }
     if warm:
         out["config"]["warm_start"] = warm["info"]
         out["config"]["warm_over_cold"] = round(value / value_cold, 3)
+    out["config"]["audit"] = audit(S, cfg, b, res)
+    log(f"bench: audit {out['config']['audit']}")
     if not args.no_cpu:
-        threads = min(16, len(os.sched_getaffinity(0)))  # one GPU's share of the host
-        cb = cpu_baseline(cfg, res_cold["iters"], args.max_iter, threads, args.cpu_budget)
+        # the host cores the GPU box allots to one GPU (OMP_NUM_THREADS there;
+        # the whole host may show more): every one of them
+        affinity = len(os.sched_getaffinity(0))
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
+        threads = max(1, min(threads, affinity))
+        cb = cpu_baseline(cfg, res_cold["iters"], args, threads, args.cpu_budget)
+        c1 = config1_single_thread(S, WL)
         out["cpu_baseline"] = {
             "value": round(cb["value"], 4), "unit": "solves/s", "cores": threads,
             "kind": "port",
@@ -388,8 +495,11 @@ def main():
                       f"oracle (same mixer, options, tolerance) in {cb['seconds']:.1f} s on "
                       f"{threads} threads, {cb['converged']} converged; full batch "
                       f"{cb['full_time']:.0f} s by work ratio sum(iters*n*N)",
+            "host_cpu_count": os.cpu_count(), "host_affinity": affinity,
+            "gpu_over_cpu": round(value / cb["value"], 1),
+            "config1_single_thread": c1,
         }
-        log(f"bench: cpu baseline {cb}")
+        log(f"bench: cpu baseline {cb}; config 1 {c1}")
     b.close()
     print(json.dumps(out), flush=True)
 

@@ -4,8 +4,10 @@ A dense network (scft_amd.Mlp: FP32, trained on the device by the library's
 own kernels) maps an instance's parameters and cold-start field to its
 converged field w(x):
 
-  inputs   L, log N, log m (domain length, contour steps, elements) and the
-           cold-start field w0 sampled on the 128-point grid  -> 131 values
+  inputs   L, log N, log m, log h_first, log h_last (domain length, contour
+           steps, elements, first and last element: the two ends of a mesh
+           graded toward the wall, equal on a uniform one) and the cold-start
+           field w0 sampled on the 128-point grid             -> 133 values
   outputs  w on the 128-point grid, and w at the nodes of index NODE_IDX in
            the scale asinh(w / W_SCALE)                     -> 136 values
   grid     96 points uniform on the wall layer 0 <= x <= 1.2 (absolute: the
@@ -45,7 +47,8 @@ NODE_IDX = np.array([1, 2, 3, 4, 6, 8, 12, 16])  # node samples near the wall
 X_SPIKE = 0.04      # nodes closer to the wall than this use the node samples
 W_SCALE = 10.0      # node-sample scale: asinh(w / W_SCALE)
 CLIP = 50.0         # guard on the smooth part of the targets
-DIMS = [3 + GRID, 256, 256, 256, 256, GRID + len(NODE_IDX)]
+N_SCALARS = 5       # L, log N, log m, log h_first, log h_last
+DIMS = [N_SCALARS + GRID, 256, 256, 256, 256, GRID + len(NODE_IDX)]
 
 # node k = 1 .. NODE_IDX[-1] from the samples, linear in the index (an
 # exponential decay in x is linear in the asinh scale)
@@ -80,14 +83,16 @@ def spike_nodes(x: np.ndarray) -> int:
 
 
 def features(S, xs, Ns, w0s) -> np.ndarray:
-    """Network inputs (n x 131) of instances with nodes xs, contour steps Ns
+    """Network inputs (n x 133) of instances with nodes xs, contour steps Ns
     and cold-start fields w0s (all nodes)."""
     Ls = np.array([x[-1] for x in xs], dtype=np.float64)
-    F = np.empty((len(xs), 3 + GRID))
+    F = np.empty((len(xs), N_SCALARS + GRID))
     F[:, 0] = Ls
     F[:, 1] = np.log(np.asarray(Ns, dtype=np.float64))
     F[:, 2] = np.log(np.array([len(x) - 1 for x in xs], dtype=np.float64))
-    F[:, 3:] = S.resample_spline_batch(xs, [_extend_to_wall(x, w, 0) for x, w in zip(xs, w0s)],
+    F[:, 3] = np.log(np.array([x[1] - x[0] for x in xs], dtype=np.float64))
+    F[:, 4] = np.log(np.array([x[-1] - x[-2] for x in xs], dtype=np.float64))
+    F[:, N_SCALARS:] = S.resample_spline_batch(xs, [_extend_to_wall(x, w, 0) for x, w in zip(xs, w0s)],
                                        grid(Ls)).reshape(len(xs), GRID)
     return F
 
@@ -136,7 +141,7 @@ def normalisation(X: np.ndarray, Y: np.ndarray):
     a per-feature std would blow up to unit size)."""
     xm, xs = X.mean(axis=0), X.std(axis=0)
     ym, ys = Y.mean(axis=0), Y.std(axis=0)
-    xs[3:] = 1.0
+    xs[N_SCALARS:] = 1.0
     xs[xs < 1e-6] = 1.0
     ys[ys < 1e-6] = 1.0
     return xm, xs, ym, ys

@@ -1,7 +1,8 @@
 """Warm-start MLP: the device FP32 kernels (matrix-core GEMMs, fused
 epilogues, MSE, Adam) against the FP64 restatement in oracle/oracle_mlp.c
-(itself pinned by finite differences), on the production shape
-131 -> 4 x 256 tanh -> 128 and on small ragged shapes."""
+(itself pinned by finite differences), on the network the warm start trains
+(scft_amd.warmstart.DIMS: 133 -> 4 x 256 tanh -> 136), on the configuration-4
+shape 131 -> 4 x 256 tanh -> 128 and on small ragged shapes."""
 import numpy as np
 import pytest
 
@@ -9,7 +10,18 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [[131, 256, 256, 256, 256, 128], [5, 70, 3], [64, 64, 64]]
+def _warm_dims():
+    import importlib.util
+    import os
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "scft_amd",
+                        "warmstart.py")
+    spec = importlib.util.spec_from_file_location("scft_warmstart_dims", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return list(mod.DIMS)
+
+
+SHAPES = [_warm_dims(), [131, 256, 256, 256, 256, 128], [5, 70, 3], [64, 64, 64]]
 
 
 def _S():
