@@ -193,8 +193,15 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
         log(f"bench: warm step {i}: {t_pred:.3f} s predict + {r['seconds']:.3f} s solve, "
             f"{r['sweeps']} sweeps, {res['converged']}/{len(cfg['xs'])} converged, mean iters "
             f"{float(np.mean(res['iters'])):.1f}, max {int(np.max(res['iters']))}, {fb} fallbacks")
+    # first residuals of the predicted and the cold fields (outside the timed steps)
+    first_warm = b.eval(b.split(ws.predict_unknowns(prep)))["rnorm"]
+    first_cold = b.eval(w0)["rnorm"]
+    q = lambda a: [float(f"{v:.3g}") for v in np.quantile(a, [0.1, 0.5, 0.9])]
+    log(f"bench: first residual quantiles 10/50/90%: warm {q(first_warm)}, cold {q(first_cold)}")
     ws.close()
     info = {
+        "first_residual_q10_q50_q90": q(first_warm),
+        "cold_first_residual_q10_q50_q90": q(first_cold),
         "network": f"{WS.DIMS[0]} -> 4 x 256 tanh -> {WS.DIMS[-1]}, FP32, device "
                    "forward/backward/Adam kernels",
         "training_fields": int(len(X)), "epochs": 50, "batch": 256, "lr": 1e-3,
