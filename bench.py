@@ -21,9 +21,10 @@ until its residual is below 1e-6.  Two legs are timed, K steps each:
   warm   every instance starts from the prediction of the warm-start MLP,
          trained (untimed setup) on fields that the batched solver itself
          converged for instances of the same workload drawn from disjoint
-         seeds; the timed step includes the prediction (network forward on
-         the device, spline resampling onto every mesh); an instance whose
-         first residual exceeds 10x the batch median restarts from w0
+         seeds; the timed step includes the prediction (network forward and
+         the mapping of its outputs onto every mesh, on the device, straight
+         into the resident field); an instance whose first residual exceeds
+         10x the batch median restarts from w0
 
 `value` = warm-start instances solved per second (cold with --no-warm);
 config.cold_start holds the cold leg and config.warm_over_cold the ratio.
@@ -138,9 +139,9 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     """Train the warm-start network on fields the batched solver converges
     for `args.train_inst` instances drawn like the bench batch from disjoint
     seeds (setup, untimed), then time K warm-started solves of the bench
-    batch.  A timed step covers the prediction (network forward on the
-    device, spline resampling onto every mesh, the upload of the predicted
-    fields) and the device-resident solve;
+    batch.  A timed step covers the prediction (network forward and the
+    mapping of its outputs onto every mesh, both on the device, writing the
+    batch's resident field) and the device-resident solve;
     an instance whose first residual exceeds 10x the batch median restarts
     from w0."""
     from scft_amd import warmstart as WS
@@ -171,11 +172,12 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
     w0_flat = b.join(w0)
+    b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
+    plan = ws.device_plan(b, prep)  # network inputs and mapping tables, resident in HBM
 
     def step():
         t = time.perf_counter()
-        wi = ws.predict_unknowns(prep)
-        b.upload(wi, w0_flat)  # the predicted fields (and the fallback's) into HBM
+        plan.predict()  # forward + mapping kernel: writes the batch's field in HBM
         t_pred = time.perf_counter() - t
         r = b.solve_resident(wopts)
         return t_pred, r
@@ -194,8 +196,10 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
             f"{r['sweeps']} sweeps, {res['converged']}/{len(cfg['xs'])} converged, mean iters "
             f"{float(np.mean(res['iters'])):.1f}, max {int(np.max(res['iters']))}, {fb} fallbacks")
     # first residuals of the predicted and the cold fields (outside the timed steps)
-    first_warm = b.eval(b.split(ws.predict_unknowns(prep)))["rnorm"]
+    plan.predict()
+    first_warm = b.eval(b.download()["w"])["rnorm"]
     first_cold = b.eval(w0)["rnorm"]
+    plan.close()
     q = lambda a: [float(f"{v:.3g}") for v in np.quantile(a, [0.1, 0.5, 0.9])]
     log(f"bench: first residual quantiles 10/50/90%: warm {q(first_warm)}, cold {q(first_cold)}")
     ws.close()

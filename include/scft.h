@@ -371,6 +371,28 @@ int scft_mlp_train(scft_mlp *m, int n_samples, const float *X, const float *Y,
                    int epochs, int batch, float lr, unsigned long long seed,
                    double *epoch_loss);
 
+/* Warm start written into a batch on the device (no reference counterpart):
+ * the network's forward pass on inputs kept in HBM, then one kernel that maps
+ * each instance's outputs onto its unknowns and writes them into the batch's
+ * resident field (what scft_batch_upload's w would have written), so no field
+ * crosses PCIe between the prediction and scft_batch_solve_resident.  Output
+ * row of instance k (after the network's output normalisation): n_grid
+ * values at the points grid_x[k * n_grid ...] (increasing), interpolated by
+ * the not-a-knot spline of scft_resample_spline, then n_node samples
+ * asinh(w / node_scale) at the 1-based node numbers node_idx (increasing);
+ * the first wall_count[k] unknowns (nodes 1..wall_count[k], at most
+ * node_idx[n_node-1]) take the samples interpolated linearly in the node
+ * number.  inputs: n_inst rows of n_in raw features (normalised here with the
+ * network's input normalisation as it stands: create the plan after
+ * training).  n_grid <= 256, n_node <= 32, dims[last] = n_grid + n_node.
+ * The plan keeps pointers to the batch and the network: destroy it first. */
+typedef struct scft_warm_plan scft_warm_plan;
+int scft_warm_plan_create(scft_warm_plan **out, scft_batch *b, scft_mlp *net, int n_in,
+                          const float *inputs, int n_grid, const double *grid_x, int n_node,
+                          const int *node_idx, double node_scale, const int *wall_count);
+int scft_warm_plan_predict(scft_warm_plan *p); /* synchronous on the batch's stream */
+int scft_warm_plan_destroy(scft_warm_plan *p);
+
 /* Cubic spline through (x_src, y_src) (x increasing, n_src >= 4) with the
  * not-a-knot end conditions of DEALII_SCFT/src/spline_chen.c:13-107 (the
  * reference's m == NULL case), evaluated at x_dst; points outside the table

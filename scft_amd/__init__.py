@@ -132,6 +132,10 @@ def _load():
     L.scft_mlp_set_normalisation.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p]
     L.scft_mlp_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
+    L.scft_warm_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_int,
+                                        _fp, C.c_int, _dp, C.c_int, _ip, C.c_double, _ip]
+    L.scft_warm_plan_predict.argtypes = [C.c_void_p]
+    L.scft_warm_plan_destroy.argtypes = [C.c_void_p]
     L.scft_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.POINTER(C.c_double),
                                      C.c_void_p]
     L.scft_mlp_adam_step.argtypes = [C.c_void_p, C.c_float]
@@ -386,6 +390,47 @@ class Batch:
         out = np.empty((N // 2 - 1) * n)
         check(lib.scft_debug_slabs(self.h, inst, out), "scft_debug_slabs")
         return out.reshape(N // 2 - 1, n)
+
+
+# ---------------------------------------------------------------- warm plan
+class WarmPlan:
+    """The warm start's prediction written straight into a batch's resident
+    field (scft_warm_plan_*): the network's forward on inputs kept in HBM,
+    then one kernel mapping each instance's outputs onto its unknowns.
+
+    inputs: n_inst x n_in raw features; grid_x: n_inst x n_grid points;
+    node_idx: the 1-based node numbers of the wall samples, node_scale their
+    asinh scale; wall_count: per instance, the leading unknowns taken from
+    the samples."""
+
+    def __init__(self, batch, mlp, inputs, grid_x, node_idx, node_scale, wall_count):
+        inputs = _f32(inputs)
+        grid_x = _f64(grid_x)
+        node_idx = np.ascontiguousarray(node_idx, dtype=np.int32)
+        wall_count = np.ascontiguousarray(wall_count, dtype=np.int32)
+        if inputs.ndim != 2 or inputs.shape[0] != len(batch.n) or grid_x.ndim != 2 or \
+                grid_x.shape[0] != len(batch.n) or len(wall_count) != len(batch.n):
+            raise ValueError("one row of inputs, grid points and wall count per instance")
+        self._keep = (batch, mlp)
+        h = C.c_void_p()
+        check(lib.scft_warm_plan_create(C.byref(h), batch.h, mlp.h, inputs.shape[1], inputs,
+                                        grid_x.shape[1], grid_x, len(node_idx), node_idx,
+                                        float(node_scale), wall_count), "scft_warm_plan_create")
+        self.h = h
+
+    def predict(self):
+        check(lib.scft_warm_plan_predict(self.h), "scft_warm_plan_predict")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib.scft_warm_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- resampling

@@ -19,7 +19,7 @@ OBJDIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
 SOURCES = ["host_math.cpp", "march.hip", "irk4.hip", "anderson.hip", "mlp.hip", "capi.hip",
-           "broyden.cpp", "noalloc.hip"]
+           "broyden.cpp", "noalloc.hip", "warmpredict.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
           f"--offload-arch={ARCH}",

@@ -967,6 +967,118 @@ extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, doub
   return rc;
 }
 
+// ------------------------------------------- warm start into the batch
+// The network's outputs mapped onto the batch's unknowns on the device
+// (warmpredict.hip): the prediction writes the resident field directly.
+struct scft_warm_plan {
+  scft_batch* b = nullptr;
+  scft_mlp* net = nullptr;
+  int n_inst = 0, n_in = 0, n_out = 0, n_grid = 0, n_node = 0;
+  double node_scale = 1.0;
+  float* d_xn = nullptr;     // normalised inputs, n_inst x n_in
+  double* d_grid = nullptr;  // n_inst x n_grid
+  int* d_node_idx = nullptr;
+  int* d_wall = nullptr;
+  float* d_norm = nullptr;   // output shift, then scale (2 n_out)
+};
+
+extern "C" int scft_warm_plan_destroy(scft_warm_plan* p) {
+  if (!p) return SCFT_E_ARG;
+  dfree(p->d_xn);
+  dfree(p->d_grid);
+  dfree(p->d_node_idx);
+  dfree(p->d_wall);
+  dfree(p->d_norm);
+  delete p;
+  return SCFT_OK;
+}
+
+extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_mlp* net,
+                                     int n_in, const float* inputs, int n_grid,
+                                     const double* grid_x, int n_node, const int* node_idx,
+                                     double node_scale, const int* wall_count) {
+  if (!out || !b || !net || !inputs || !grid_x || !wall_count || n_in < 1) return SCFT_E_ARG;
+  if (n_grid < 4 || n_grid > WARM_MAX_GRID || n_node < 0 || n_node > WARM_MAX_NODES ||
+      (n_node > 0 && (!node_idx || n_node < 2)))
+    return SCFT_E_ARG;
+  *out = nullptr;
+  for (int j = 0; j < n_node; j++)
+    if (node_idx[j] < 1 || (j > 0 && node_idx[j] <= node_idx[j - 1])) return SCFT_E_ARG;
+  for (int k = 0; k < b->n_inst; k++) {
+    if (wall_count[k] < 0 || wall_count[k] > b->desc[k].n ||
+        (wall_count[k] > 0 && (n_node < 2 || wall_count[k] > node_idx[n_node - 1])))
+      return SCFT_E_ARG;
+    const double* g = grid_x + static_cast<size_t>(k) * n_grid;
+    for (int i = 1; i < n_grid; i++)
+      if (!(g[i] > g[i - 1])) return SCFT_E_ARG;
+  }
+  auto* p = new scft_warm_plan();
+  p->b = b;
+  p->net = net;
+  p->n_inst = b->n_inst;
+  p->n_in = n_in;
+  p->n_grid = n_grid;
+  p->n_node = n_node;
+  p->n_out = n_grid + n_node;
+  p->node_scale = node_scale;
+  std::vector<float> xn(static_cast<size_t>(p->n_inst) * n_in);
+  int rc = scft_mlp_normalise_inputs(net, inputs, p->n_inst, xn.data());
+  auto fail = [&](int code) {
+    scft_warm_plan_destroy(p);
+    return code;
+  };
+  if (rc) return fail(rc);
+  if ((rc = dalloc(&p->d_xn, xn.size())) || (rc = dalloc(&p->d_grid, (size_t)p->n_inst * n_grid)) ||
+      (rc = dalloc(&p->d_node_idx, std::max(1, n_node))) || (rc = dalloc(&p->d_wall, p->n_inst)) ||
+      (rc = dalloc(&p->d_norm, 2 * static_cast<size_t>(p->n_out))))
+    return fail(rc);
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream) == hipSuccess;
+  };
+  if (!up(p->d_xn, xn.data(), sizeof(float) * xn.size()) ||
+      !up(p->d_grid, grid_x, sizeof(double) * p->n_inst * n_grid) ||
+      (n_node > 0 && !up(p->d_node_idx, node_idx, sizeof(int) * n_node)) ||
+      !up(p->d_wall, wall_count, sizeof(int) * p->n_inst) ||
+      hipStreamSynchronize(b->stream) != hipSuccess)
+    return fail(SCFT_E_HIP);
+  *out = p;
+  return SCFT_OK;
+}
+
+extern "C" int scft_warm_plan_predict(scft_warm_plan* p) {
+  if (!p) return SCFT_E_ARG;
+  scft_batch* b = p->b;
+  const float *d_y = nullptr, *ym = nullptr, *ys = nullptr;
+  int n_out = 0, n_in = 0;
+  SCFT_TRY(scft_mlp_forward_resident(p->net, p->n_inst, p->d_xn, &d_y, &ym, &ys, &n_out, &n_in));
+  if (n_out != p->n_out || n_in != p->n_in) return SCFT_E_ARG;
+  const bool norm = ym && ys;
+  if (norm) {
+    HIPCHK(hipMemcpyAsync(p->d_norm, ym, sizeof(float) * n_out, hipMemcpyHostToDevice,
+                          b->stream));
+    HIPCHK(hipMemcpyAsync(p->d_norm + n_out, ys, sizeof(float) * n_out, hipMemcpyHostToDevice,
+                          b->stream));
+  }
+  WarmMapArgs a;
+  a.desc = b->d_desc;
+  a.x = b->d_x;
+  a.w = b->d_w;
+  a.y = d_y;
+  a.ym = norm ? p->d_norm : nullptr;
+  a.ys = norm ? p->d_norm + n_out : nullptr;
+  a.grid = p->d_grid;
+  a.node_idx = p->d_node_idx;
+  a.wall_count = p->d_wall;
+  a.node_scale = p->node_scale;
+  a.n_inst = p->n_inst;
+  a.n_out = n_out;
+  a.n_grid = p->n_grid;
+  a.n_node = p->n_node;
+  HIPCHK(launch_warm_map(a, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return SCFT_OK;
+}
+
 // ------------------------------------------------------- drop-in evaluator
 extern "C" {
 double* f0_given = nullptr;

@@ -423,6 +423,33 @@ extern "C" int scft_mlp_forward(scft_mlp* m, int batch, const float* X, float* Y
   return SCFT_OK;
 }
 
+// Forward pass for the device-side warm-start prediction (scft_internal.h).
+int scft_mlp_forward_resident(scft_mlp* m, int batch, const float* d_xn, const float** d_out,
+                              const float** ym, const float** ys, int* n_out, int* n_in) {
+  if (!m || batch < 1 || !d_xn || !d_out) return SCFT_E_ARG;
+  if (int rc = ensure_batch(m, batch)) return rc;
+  const int din = m->dims.front();
+  MLPCHK(hipMemcpyAsync(m->d_act[0], d_xn, sizeof(float) * (size_t)batch * din,
+                        hipMemcpyDeviceToDevice, m->stream));
+  if (int rc = forward(m, batch)) return rc;
+  MLPCHK(hipStreamSynchronize(m->stream));
+  *d_out = m->d_act.back();
+  if (ym) *ym = m->ym.empty() ? nullptr : m->ym.data();
+  if (ys) *ys = m->ys.empty() ? nullptr : m->ys.data();
+  if (n_out) *n_out = m->dims.back();
+  if (n_in) *n_in = din;
+  return SCFT_OK;
+}
+
+// x' = (x - xm) / xs of `rows` host rows, as scft_mlp_forward applies it.
+int scft_mlp_normalise_inputs(const scft_mlp* m, const float* X, int rows, float* out) {
+  if (!m || !X || !out || rows < 0) return SCFT_E_ARG;
+  std::vector<float> xn;
+  normalise_rows(m, X, rows, xn);
+  std::copy(xn.begin(), xn.end(), out);
+  return SCFT_OK;
+}
+
 // Loss and gradient on raw (already normalised) rows, for gradient checks:
 // no normalisation is applied here.
 extern "C" int scft_mlp_loss_grad(scft_mlp* m, int batch, const float* X, const float* T,

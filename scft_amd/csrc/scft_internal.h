@@ -117,4 +117,42 @@ hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_reset(int* counters, int n, hipStream_t s);
 constexpr int MIX_MAX_HISTORY = 16;
 
+// Mapping of the warm-start network's outputs onto a batch's unknowns
+// (warmpredict.hip): instance k's row of raw outputs y (before the output
+// normalisation y' ys + ym) holds n_grid values at its grid points, then
+// n_node wall samples asinh(w / node_scale) at the 1-based node numbers
+// node_idx; its first wall_count[k] unknowns come from the samples
+// (linear in the node number), the others from the not-a-knot cubic spline
+// through the grid values.  One workgroup per instance.
+constexpr int WARM_MAX_GRID = 256;
+constexpr int WARM_MAX_NODES = 32;
+struct WarmMapArgs {
+  const InstDesc* desc;
+  const double* x;         // the batch's node coordinates
+  double* w;               // out: the batch's field on unknowns
+  const float* y;          // network outputs, n_inst x n_out
+  const float* ym;         // output shift and scale (nullable: none)
+  const float* ys;
+  const double* grid;      // n_inst x n_grid grid points, increasing
+  const int* node_idx;     // n_node sample node numbers, increasing
+  const int* wall_count;   // per instance
+  double node_scale;
+  int n_inst, n_out, n_grid, n_node;
+};
+hipError_t launch_warm_map(const WarmMapArgs& a, hipStream_t s);
+
+}  // namespace scft
+
+// The network's forward pass on rows already normalised and resident in
+// device memory (mlp.hip): *d_out receives its raw output activations
+// (batch x dims.back(), before the output normalisation), *ym / *ys its
+// output shift and scale as host arrays (nullptr: none).  Waits for the
+// network's stream.
+struct scft_mlp;
+int scft_mlp_forward_resident(scft_mlp* m, int batch, const float* d_xn, const float** d_out,
+                              const float** ym, const float** ys, int* n_out, int* n_in);
+int scft_mlp_normalise_inputs(const scft_mlp* m, const float* X, int rows, float* out);
+
+namespace scft {
+
 }  // namespace scft

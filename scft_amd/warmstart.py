@@ -197,6 +197,16 @@ class WarmStart:
         out[prep.sp_dst] = spike[prep.sp_inst, prep.sp_node]
         return out
 
+    def device_plan(self, batch, prep: Prepared):
+        """A plan that writes the prediction for `prep`'s configuration
+        straight into `batch`'s resident field (scft_amd.WarmPlan): the
+        same mapping as predict_unknowns, on the device.  Create it after
+        training; close it before the batch or the network."""
+        counts = np.zeros(len(prep.gn), dtype=np.int32)
+        for i, _, k in prep.spikes:
+            counts[i] = k
+        return self.S.WarmPlan(batch, self.mlp, prep.F, prep.grids, NODE_IDX, W_SCALE, counts)
+
     def close(self):
         self.mlp.close()
 
