@@ -221,6 +221,87 @@ def solver_opts(S, args):
                            progress_seconds=20.0, **solver_settings(args.workload))
 
 
+def continuation_compare(args):
+    """The reference's own warm start, mesh continuation (DEALII_SCFT/
+    drivescft.cc:291-322: solve on a coarser mesh, carry the field to the
+    finer one by spline, solve there), on the bench batch: every mesh is
+    coarsened to every other node (the mesh whose uniform bisection gives it
+    back), the coarse batch is solved from w0, its fields are moved onto the
+    bench meshes by the not-a-knot spline (scft_resample_spline_batch, as
+    spline_chen.c), and the bench batch is solved from them with the cold
+    fallback.  Timed per step: both solves and the transfer.  The cold leg is
+    timed beside it; the MLP leg is the bench line's own on the same batch."""
+    import scft_amd as S
+    from scft_amd import workloads as WL
+    if args.n_inst <= 0:
+        args.n_inst = WORKLOADS[args.workload]["n_inst"]
+    cfg = make_workload(WL, args.workload, args.n_inst)
+    xs = cfg["xs"]
+    xc = [np.concatenate([x[:-1:2], x[-1:]]) if (len(x) - 1) % 2 else x[::2] for x in xs]
+    xc = [x if len(x) >= 5 else xs[i] for i, x in enumerate(xc)]
+    w0 = [w[1:] for w in cfg["w0"]]
+    w0c = [WL.w0_field(x, WORKLOADS[args.workload]["seed"] + 1 + k)[1:] for k, x in enumerate(xc)]
+    fine = S.Batch(xs, cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN)
+    coarse = S.Batch(xc, cfg["Ns"], [WL.exp_target(x)[1:] for x in xc], bc=S.BC_DN)
+    for bb, w in ((fine, w0), (coarse, w0c)):
+        bb.upload(w)
+        bb.autotune(reps=2)
+    opts = solver_opts(S, args)
+    wopts = solver_opts(S, args)
+    wopts.fallback_factor = 10.0
+    w0_flat = fine.join(w0)
+    # the spline transfer: coarse unknowns (with the wall node) -> fine unknowns
+    src_x = [x for x in xc]
+    dst_x = [x[1:] for x in xs]
+
+    def cont_step():
+        t0 = time.perf_counter()
+        coarse.upload(w0c)
+        rc = coarse.solve_resident(opts)
+        resc = coarse.download()
+        t1 = time.perf_counter()
+        src_y = [np.concatenate([[0.0], v]) for v in resc["w"]]
+        wt = S.resample_spline_batch(src_x, src_y, dst_x, flat=True)[0]
+        fine.upload(wt, w0_flat)
+        t2 = time.perf_counter()
+        rf = fine.solve_resident(wopts)
+        t3 = time.perf_counter()
+        return t3 - t0, (t1 - t0, t2 - t1, rf["seconds"]), resc, rc
+
+    def cold_step():
+        fine.upload(w0)
+        r = fine.solve_resident(opts)
+        return r["seconds"]
+
+    for _ in range(args.warmup):
+        cont_step()
+        cold_step()
+    tc, tw, parts = 0.0, 0.0, np.zeros(3)
+    for i in range(args.steps):
+        tw += cold_step()
+        res_cold = fine.download()
+        dt, pp, resc, rc = cont_step()
+        res = fine.download()
+        tc += dt
+        parts += pp
+        log(f"bench: continuation step {i}: coarse solve {pp[0]:.3f} s ({rc['sweeps']} sweeps, "
+            f"mean iters {resc['iters'].mean():.1f}), transfer {pp[1]:.3f} s, fine solve "
+            f"{pp[2]:.3f} s ({res['sweeps']} sweeps, mean iters {res['iters'].mean():.1f}, "
+            f"max {res['iters'].max()}); cold {res_cold['iters'].mean():.1f} mean iters")
+    out = {"workload": WORKLOADS[args.workload]["text"].format(n=args.n_inst),
+           "cold_solves_per_s": round(args.n_inst * args.steps / tw, 1),
+           "continuation_solves_per_s": round(args.n_inst * args.steps / tc, 1),
+           "continuation_seconds_coarse_transfer_fine": [round(v / args.steps, 4) for v in parts],
+           "continuation_fine_mean_iters": round(float(res["iters"].mean()), 2),
+           "continuation_fine_max_iters": int(res["iters"].max()),
+           "continuation_coarse_mean_iters": round(float(resc["iters"].mean()), 2),
+           "cold_mean_iters": round(float(res_cold["iters"].mean()), 2),
+           "converged": int(res["converged"])}
+    fine.close()
+    coarse.close()
+    print(json.dumps(out), flush=True)
+
+
 def stepper_compare(args):
     """Sweep time of 4096 copies of the field the reference's deal.II IRK
     solver converged (DEALII_SCFT/inputFiles/N=33_for_read.txt, consistent
@@ -357,6 +438,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="run a plain device copy of MIB MiB 5 times and exit (counter scale)")
     ap.add_argument("--tune-file", default=None)
+    ap.add_argument("--continuation-compare", action="store_true",
+                    help="the reference's mesh continuation as the warm start of the bench "
+                         "batch (coarse solve + spline transfer + solve), against the cold "
+                         "start (one JSON line, not the bench line)")
     ap.add_argument("--stepper-compare", action="store_true",
                     help="time one sweep of 4096 copies of the reference's 33-node deal.II "
                          "problem with implicit Euler and with IRK4 (no JSON line)")
@@ -368,6 +453,9 @@ def main():
 
     if args.stepper_compare:
         stepper_compare(args)
+        return
+    if args.continuation_compare:
+        continuation_compare(args)
         return
 
     if args.copy_calibration:
