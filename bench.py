@@ -87,6 +87,25 @@ WORKLOADS = {
 }
 
 
+# The warm-start form of the preconditioner used from predicted starts:
+# g = f + 0.5 d^-1 (M_L^-1 K + 4 diag(max(w, 0))) (d f), d = sqrt(phi_t).
+# From exact converged fields passed through the network's output
+# representation (the best a prediction can be) it needs 7.1 evaluations on
+# average on the production mix and 5.7 on configuration 3, against 16.6 and
+# 11.2 with the cold form (CPU oracle, 24 instances each); from a cold start
+# it does not converge, so the cold leg keeps the cold form.
+WARM_PRECOND = 0.5
+WARM_FIELD_WEIGHT = 4.0
+# Some predicted starts (stiff graded meshes with N = 512 mostly) stall or
+# diverge in the warm form; an instance not converged after 24 evaluations
+# goes over to the cold form with its current field (history cleared).  On
+# the 16384-instance production mix that is ~3% of the instances, and the
+# batch time is lowest there (16: 1.37 s, 20: 1.41 s, 24: 1.33 s per solve;
+# without the switch ~1% never converge); on the 4096-instance mix no
+# instance needs it (the slowest converges in 21).
+WARM_SWITCH = 24
+
+
 def solver_settings(workload):
     return dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8,
                 precond=WORKLOADS[workload]["precond"], backtrack=1e30, max_halvings=30,
@@ -95,8 +114,10 @@ def solver_settings(workload):
 
 def solver_text(workload):
     return (f"device Anderson, history 16, log mixing preconditioned by I + "
-            f"{WORKLOADS[workload]['precond']} M^-1 K, beta 1.0, no step cap, NaN "
-            "backtracking, field bounded below by -0.5/dt")
+            f"{WORKLOADS[workload]['precond']} M^-1 K (cold) or I + {WARM_PRECOND} "
+            f"d^-1 (M^-1 K + {WARM_FIELD_WEIGHT} W+) d, d = sqrt(phi_t) (predicted starts, "
+            f"cold form after {WARM_SWITCH} evaluations), "
+            "beta 1.0, no step cap, NaN backtracking, field bounded below by -0.5/dt")
 
 
 def make_workload(WL, workload, n_inst, seed=None):
@@ -171,6 +192,12 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     prep = WS.Prepared(S, cfg)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
+    # from a predicted start: the warm-start form of the preconditioner
+    # (q-weighted Laplacian screened by the local field); instances that fall
+    # back to w0 mix with the cold form
+    wopts.warm_precond = WARM_PRECOND
+    wopts.warm_field_weight = WARM_FIELD_WEIGHT
+    wopts.warm_switch = WARM_SWITCH
     w0_flat = b.join(w0)
     b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
     plan = ws.device_plan(b, prep)  # network inputs and mapping tables, resident in HBM

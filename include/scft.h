@@ -53,6 +53,7 @@ extern "C" {
 #define SCFT_ST_MAXITER 8u
 #define SCFT_ST_FALLBACK 16u /* warm start rejected, restarted cold */
 #define SCFT_ST_FAILED 32u   /* evaluation failed (NaN / pivot) beyond recovery */
+#define SCFT_ST_COLD_FORM 64u /* left the warm-start preconditioner (warm_switch) */
 
 /* ---- library lifetime ---------------------------------------------------
  * Replaces PetscInitialize/PetscFinalize around the evaluator
@@ -181,6 +182,22 @@ typedef struct scft_anderson_opts {
                              instance), so 1 + dt w >= 1 - field_floor and the
                              implicit-Euler matrix stays strictly diagonally
                              dominant (0: off, the reference's update) */
+  double warm_precond;    /* a_w > 0: instances that start from a predicted
+                             field (and do not fall back to w_cold) mix
+                             g = f + a_w d^-1 (M_L^{-1} K + c diag(max(w,0)))
+                             (d f), d = sqrt(max(phi_t, floor)): the
+                             Laplacian acting on q-weighted residuals (q ~
+                             sqrt(phi) near the wall) screened by the local
+                             field.  Converges in half the evaluations from
+                             starts near the solution; from a cold start it
+                             does not (0: off) */
+  double warm_field_weight; /* c of warm_precond (4) */
+  int warm_switch;          /* an instance leaves the warm form for the plain
+                               one (history cleared, SCFT_ST_COLD_FORM) after
+                               this many evaluations without converging
+                               (0: never) ... */
+  double warm_switch_growth;/* ... or when its ||r||_inf exceeds this factor
+                               times its best so far (0: never) */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

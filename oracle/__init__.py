@@ -60,7 +60,9 @@ class AndersonOpts(C.Structure):
                 ("restart_factor", C.c_double), ("regularization", C.c_double),
                 ("transform", C.c_int), ("floor", C.c_double), ("precond", C.c_double),
                 ("pstencil", C.c_void_p), ("backtrack", C.c_double), ("max_halvings", C.c_int),
-                ("field_floor", C.c_double), ("field_lower", C.c_double)]
+                ("field_floor", C.c_double), ("field_lower", C.c_double),
+                ("warm_precond", C.c_double), ("warm_field_weight", C.c_double),
+                ("warm_switch", C.c_int), ("warm_switch_growth", C.c_double)]
 
 
 class AndersonStats(C.Structure):
@@ -315,10 +317,12 @@ MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=2000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
-                  precond=0.0, backtrack=0.0, max_halvings=30, field_floor=0.0, field_lower=0.0):
+                  precond=0.0, backtrack=0.0, max_halvings=30, field_floor=0.0, field_lower=0.0,
+                  warm_precond=0.0, warm_field_weight=4.0, warm_switch=0, warm_switch_growth=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, precond, None, backtrack, max_halvings, field_floor,
-                        field_lower)
+                        field_lower, warm_precond, warm_field_weight, warm_switch,
+                        warm_switch_growth)
 
 
 def solve_anderson(x, phi_t, N, w_full, opts=None, bc=BC_DN, cmat=C_ROWSCALED, trace_len=0):

@@ -69,7 +69,7 @@ int orc_solve_anderson(const orc_problem *p, double *w_full,
   orc_anderson_opts oo = *o;
   if (oo.field_floor > 0.0) oo.field_lower = -oo.field_floor * p->N;
   double *ps = NULL;
-  if (oo.precond > 0.0 && !oo.pstencil) { /* stencil of this instance's mesh */
+  if ((oo.precond > 0.0 || oo.warm_precond > 0.0) && !oo.pstencil) { /* this mesh's stencil */
     ps = (double *)malloc(sizeof(double) * 3 * (size_t)n);
     if (!ps) { orc_ctx_free(&c); return ORC_E_ALLOC; }
     orc_precond_stencil(p->x, p->M, p->bc, ps);

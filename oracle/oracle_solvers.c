@@ -180,7 +180,7 @@ int orc_precond_stencil(const double *x, int M, int bc, double *st) {
 int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
                  const orc_anderson_opts *o, const double *phi_t_unk,
                  orc_anderson_stats *st) {
-  if (o->precond > 0.0 && !o->pstencil) return 3;
+  if ((o->precond > 0.0 || o->warm_precond > 0.0) && !o->pstencil) return 3;
   int mh = o->history;
   int S = mh + 1;
   double *wh = (double *)calloc((size_t)n * S, sizeof(double));
@@ -191,6 +191,7 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
   double U[64 * 64], bb[64];
   int stored = 0, head = 0, k, rc = 1, restarts = 0, halvings = 0;
   double best = INFINITY, err = INFINITY, eacc = INFINITY;
+  int warm_on = o->warm_precond > 0.0;
   for (k = 0; k < o->max_iter; k++) {
     int ferr = F(n, w, r, ctx);
     err = 0.0;
@@ -218,6 +219,12 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
       stored = 0;
       restarts++;
     }
+    if (warm_on && ((o->warm_switch > 0 && k + 1 > o->warm_switch) ||
+                    (o->warm_switch_growth > 0.0 && err > o->warm_switch_growth * best))) {
+      warm_on = 0; /* leave the warm-start form: plain form, history cleared */
+      stored = 0;
+      restarts++;
+    }
     if (err < best) best = err;
     /* mixing residual: r itself, r/max(phi_t, floor), or log(phi/phi_t) */
     if (o->transform != ORC_MIX_PLAIN && phi_t_unk) {
@@ -227,8 +234,22 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
         else r[i] = log(fmax(pt + r[i], o->floor * 1e-3) / pt);
       }
     }
-    /* preconditioned mixing residual g = f + a M_L^{-1} K f */
-    if (o->precond > 0.0) {
+    /* preconditioned mixing residual g = f + a M_L^{-1} K f, or its
+     * warm-start form g = f + a_w d^-1 (M_L^{-1} K + c W+) (d f) */
+    if (warm_on && phi_t_unk) {
+      const double *ps = o->pstencil;
+      double *h = (double *)malloc(sizeof(double) * n);
+      for (int i = 0; i < n; i++) h[i] = sqrt(fmax(phi_t_unk[i], o->floor)) * r[i];
+      for (int i = 0; i < n; i++) {
+        const double hl = i > 0 ? h[i - 1] : 0.0, hr = (i + 1 < n) ? h[i + 1] : 0.0;
+        double kh = ps[3 * i + 1] * h[i] - ps[3 * i] * hl - ps[3 * i + 2] * hr;
+        kh = (kh + o->warm_field_weight * fmax(w[i], 0.0) * h[i]) /
+             sqrt(fmax(phi_t_unk[i], o->floor));
+        wn[i] = r[i] + o->warm_precond * kh;
+      }
+      free(h);
+      memcpy(r, wn, sizeof(double) * n);
+    } else if (o->precond > 0.0) {
       const double *ps = o->pstencil;
       double fl = 0.0, fc = r[0];
       for (int i = 0; i < n; i++) {

@@ -45,6 +45,13 @@ typedef struct orc_anderson_opts {
   double field_lower;   /* the bound itself, -field_floor * N: set from the
                            instance's N by orc_solve_anderson (a caller of
                            orc_anderson sets it directly) */
+  double warm_precond;  /* a_w > 0: g = f + a_w d^-1 (M_L^{-1} K + c diag(max(w,0)))
+                           (d f), d = sqrt(max(phi_t, floor)), instead of the
+                           plain form (needs pstencil and phi_t_unk) */
+  double warm_field_weight; /* c */
+  int warm_switch;      /* leave the warm form (history cleared) after this
+                           many evaluations without converging (0: never) */
+  double warm_switch_growth; /* ... or when err > growth * best (0: never) */
 } orc_anderson_opts;
 
 /* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements

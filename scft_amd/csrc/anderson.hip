@@ -23,6 +23,8 @@
 //  * backtracking: a failed evaluation (NaN / zero pivot) or one whose ||r||
 //    exceeds backtrack times the last accepted one halves the step toward the
 //    last accepted iterate (ring slot head - 1) and clears the history;
+//  * a warm-start preconditioner for instances started near the solution
+//    (below, at the push of the history);
 //  * a lower bound w >= -field_floor / dt on the updated field.  Near a wall
 //    on a fine mesh the converged field spans decades (w at the first node
 //    1e6-1e8 when dt/h^2 ~ 1e4-1e5), and the preconditioned step from a smooth
@@ -147,13 +149,33 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     stored = 0;
     ms.restarts++;
   }
+  // the warm-start form of the preconditioner (below) while the instance
+  // started from a prediction and neither fell back to the cold field nor
+  // left the form: it leaves it (history cleared, status COLD_FORM) when it
+  // has used warm_switch evaluations or its error grows past
+  // warm_switch_growth times the best so far
+  bool wp = a.warm_precond > 0.0 && !(st0 & (ST_FALLBACK | ST_COLD_FORM));
+  if (wp && ((a.warm_switch > 0 && k > a.warm_switch) ||
+             (a.warm_switch_growth > 0.0 && err > a.warm_switch_growth * ms.best))) {
+    wp = false;
+    stored = 0;
+    ms.restarts++;
+    if (tid == 0) a.status[inst] = st0 | ST_COLD_FORM;
+  }
   if (err < ms.best) ms.best = err;
   const int cur = __builtin_amdgcn_readfirstlane(ms.head);
   int m = __builtin_amdgcn_readfirstlane(stored < a.history ? stored : a.history);
 
   // push (w_k, g_k) into slot `cur`
   const double* __restrict__ xn = a.x + d.xo;
-  const bool pc = a.precond > 0.0 && n > 1;
+  // the warm-start form of the preconditioner, for instances started from a
+  // prediction that did not fall back to the cold field: the stencil acts
+  // on q-weighted residuals h = d f, d = sqrt(max(phi_t, floor)) ~ q near
+  // the wall, plus the screening of the local field c max(w, 0):
+  //   g = f + a_w d^-1 (M_L^-1 K + c diag(max(w, 0))) (d f)
+  // (with d = 1 and c = 0 it is the plain form, bit for bit)
+  const double coef = wp ? a.warm_precond : a.precond;
+  const bool pc = coef > 0.0 && n > 1;
   // (whole-wave trips: the neighbours' mixing residuals come from the lanes
   // beside, and only a wave's edge lanes evaluate theirs again)
   for (int u0 = 0; u0 < n; u0 += AM_THREADS) {
@@ -161,12 +183,20 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     const bool act = u < n;
     double f = act ? mix_residual(r[u], pt[u], a.transform, a.floor) : 0.0;
     if (pc) {
-      double fl = __shfl_up(f, 1), fr = __shfl_down(f, 1);
+      const double dq = (wp && act) ? sqrt(fmax(pt[u], a.floor)) : 1.0;
+      const double hv = dq * f;
+      double fl = __shfl_up(hv, 1), fr = __shfl_down(hv, 1);
       if (act) {
         const int i = u + 1;  // node of unknown u
         const bool has_l = u > 0, has_r = u + 1 < n;
-        if (lane == 0 && has_l) fl = mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor);
-        if (lane == 63 && has_r) fr = mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor);
+        if (lane == 0 && has_l) {
+          fl = mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor);
+          if (wp) fl = sqrt(fmax(pt[u - 1], a.floor)) * fl;
+        }
+        if (lane == 63 && has_r) {
+          fr = mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor);
+          if (wp) fr = sqrt(fmax(pt[u + 1], a.floor)) * fr;
+        }
         fl = has_l ? fl : 0.0;
         fr = has_r ? fr : 0.0;
         const double hl = has_l ? xn[i] - xn[i - 1] : 0.0;
@@ -174,7 +204,9 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
         const double mm = 0.5 * (hl + hrr);
         const double cl = has_l ? (1.0 / hl) / mm : 0.0;
         const double cr = has_r ? (1.0 / hrr) / mm : 0.0;
-        f = f + a.precond * ((cl + cr) * f - cl * fl - cr * fr);
+        double kh = (cl + cr) * hv - cl * fl - cr * fr;
+        if (wp) kh = (kh + a.warm_field_weight * fmax(w[u], 0.0) * hv) / dq;
+        f = f + coef * kh;
       }
     }
     if (act) {

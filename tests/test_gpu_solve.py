@@ -109,6 +109,61 @@ def test_field_floor_trajectory_matches_oracle():
             assert dw <= 1e-8
 
 
+@pytest.mark.parametrize("K,switch", [(2, 0), (6, 0), (6, 3)])
+def test_warm_preconditioner_trajectory_matches_oracle(K, switch):
+    """The warm-start form of the preconditioner (q-weighted Laplacian plus
+    the field's screening, anderson.hip) against orc_anderson's, over K
+    updates from starts near the solution -- the oracle's converged fields
+    with 1e-3 relative bumps, the regime the form is meant for -- on a stiff
+    graded instance and two uniform ones (bar 1e-8 relative); with
+    warm_switch = 3 both leave the warm form after the third evaluation."""
+    S = _S()
+    c5 = WL.config5(n_inst=2048)
+    c3 = WL.config3(n_inst=3, seed=7, Ns=(256,))
+    xs = [c5["xs"][1658], c3["xs"][0], c3["xs"][1]]
+    Ns = [c5["Ns"][1658], 256, 256]
+    pts = [c5["pts"][1658], c3["pts"][0], c3["pts"][1]]
+    starts = []
+    for j in range(3):
+        w0 = (c5["w0"][1658] if j == 0 else c3["w0"][j - 1]).copy()
+        w0[0] = 0.0
+        sol = O.solve_anderson(xs[j], pts[j], Ns[j], w0,
+                               O.anderson_opts(**{**_base_opts(400), "tol": 1e-9,
+                                                  "precond": 0.6 if j == 0 else 0.5}),
+                               bc=O.BC_DN)
+        assert sol["rc"] == 0
+        x = xs[j]
+        bump = np.cos(np.pi * x / x[-1]) + 0.5 * np.sin(3.0 * np.pi * x / x[-1])
+        w = sol["w"] * (1.0 + 1e-3 * bump)
+        w[0] = 0.0
+        starts.append(w)
+    o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0, max_iter=K + 1,
+                              precond=0.5, field_floor=0.5, warm_precond=0.5,
+                              warm_field_weight=4.0, warm_switch=switch)
+    o_cpu.max_iter = K
+    b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
+    out = b.solve([w[1:] for w in starts], o_dev)
+    if switch:
+        assert ((out["status"] & S.ST_COLD_FORM) != 0).all()
+    b.close()
+    for j in range(3):
+        ref = O.solve_anderson(xs[j], pts[j], Ns[j], starts[j], o_cpu, bc=O.BC_DN)
+        plain = O.solve_anderson(xs[j], pts[j], Ns[j], starts[j],
+                                 O.anderson_opts(**{**_base_opts(K), "warm_precond": 0.0}),
+                                 bc=O.BC_DN)
+        dw = np.abs(out["w"][j] - ref["w"][1:]).max() / max(1.0, np.abs(ref["w"]).max())
+        print(f"instance {j} M={len(xs[j])}: {K} updates, relative |w_dev - w_cpu| = {dw:.2e}",
+              flush=True)
+        assert dw <= 1e-8
+        assert not np.array_equal(ref["w"], plain["w"])  # the warm form was in effect
+
+
+def _base_opts(K):
+    return dict(history=16, beta=1.0, tol=1e-6, max_iter=K, step_cap=0.0, restart_factor=1e30,
+                regularization=1e-14, transform=2, floor=1e-8, precond=0.5, field_floor=0.5,
+                warm_field_weight=4.0)
+
+
 def test_device_anderson_converges_config1():
     """Config 1 (m = 32, N = 2048, w0 = 0), history 10, mixing 0.1, cap 0.5:
     converges on the device and the converged field evaluates (on the CPU

@@ -167,3 +167,68 @@ def test_anderson_reference_linear_toy():
     w, calls = _affine_solve(A, b, np.array([1.0, 2.0]), history=10, beta=0.1, max_iter=4)
     assert calls[-1] <= 1e-13
     np.testing.assert_allclose(w, [-4.0, 6.0], rtol=0, atol=1e-12)
+
+
+# ------------------------------------------------------------ warm-start form
+def _mixer_run(x, pt, N, w0, **kw):
+    o = O.anderson_opts(history=16, beta=1.0, tol=0.0, step_cap=0.0, regularization=0.0,
+                        transform=O.MIX_LOG, floor=1e-8, **kw)
+    return O.solve_anderson(x, pt, N, w0, o, bc=O.BC_DN)["w"]
+
+
+def test_warm_preconditioner_reduces_to_the_plain_one():
+    """g = f + a d^-1 (K + c W+)(d f) with d constant (phi_t = 1 everywhere,
+    no wall layer) and c = 0 is the plain form g = f + a K f: the same
+    iterates bit for bit."""
+    x = O.mesh_uniform(41, 3.0)
+    pt = np.ones(41)
+    w0 = 0.3 * np.cos(2.0 * x)
+    w0[0] = 0.0
+    for k in (1, 2, 4):
+        plain = _mixer_run(x, pt, 128, w0, max_iter=k, precond=0.5)
+        warm = _mixer_run(x, pt, 128, w0, max_iter=k, precond=0.5, warm_precond=0.5,
+                          warm_field_weight=0.0)
+        assert np.array_equal(plain, warm)
+    # with the wall target the two differ (d varies next to the wall)
+    ptw = O.target_position(x, O.TAU, False)
+    a = _mixer_run(x, ptw, 128, w0, max_iter=2, precond=0.5)
+    b = _mixer_run(x, ptw, 128, w0, max_iter=2, precond=0.5, warm_precond=0.5,
+                   warm_field_weight=0.0)
+    assert not np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("n", [6, 12])
+def test_warm_preconditioner_keeps_finite_termination(n):
+    """With c = 0 the warm form is a fixed linear operator P = I + a D^-1 K D,
+    so Anderson on the preconditioned affine residual P (b - A w) is still
+    GMRES and terminates by evaluation n+2 (plain transform here: the test
+    drives orc_anderson with an affine residual, phi_t only weights d)."""
+    rng = np.random.default_rng(200 + n)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.geomspace(0.2, 2.0, n)) @ Q.T
+    b = rng.standard_normal(n)
+    x = np.linspace(0.0, 1.0, n + 1)
+    st = np.zeros(3 * n)
+    O.lib().orc_precond_stencil(np.ascontiguousarray(x), n + 1, O.BC_DN, st)
+    phi_t = np.geomspace(1e-4, 1.0, n)  # d spans two decades, as next to a wall
+    calls = []
+
+    def F(nn, w, r, ctx):
+        wv = np.ctypeslib.as_array(w, shape=(nn,))
+        rv = b - A @ wv
+        for i in range(nn):
+            r[i] = rv[i]
+        calls.append(float(np.abs(rv).max()))
+        return 0
+
+    cb = O.RESIDUAL_FN(F)
+    opts = O.anderson_opts(history=16, beta=1.0, tol=0.0, max_iter=n + 2, step_cap=0.0,
+                           regularization=0.0, transform=O.MIX_PLAIN, precond=0.0,
+                           warm_precond=0.05, warm_field_weight=0.0)
+    opts.pstencil = st.ctypes.data
+    w = np.zeros(n)
+    stt = O.AndersonStats(0, 0, 0.0, None, 0)
+    O.lib().orc_anderson(w, n, C.cast(cb, C.c_void_p), None, C.byref(opts),
+                         np.ascontiguousarray(phi_t).ctypes.data, C.byref(stt))
+    assert calls[n + 1] <= 1e-8 * np.abs(b).max(), calls
+    np.testing.assert_allclose(w, np.linalg.solve(A, b), rtol=0, atol=1e-7)
