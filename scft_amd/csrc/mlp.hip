@@ -251,12 +251,14 @@ int forward(scft_mlp* m, int batch) {
   return SCFT_OK;
 }
 
-// loss and gradient for the rows in d_act[0] against targets in d_t
-int backward(scft_mlp* m, int batch) {
+// loss and gradient for the rows in d_act[0] against targets in d_t; the
+// squared error is added to *d_loss, which is cleared first unless the
+// caller accumulates over several batches
+int backward(scft_mlp* m, int batch, bool clear_loss = true) {
   const int L = static_cast<int>(m->dims.size()) - 1;
   const int dl = m->dims[L];
   const long long n = (long long)batch * dl;
-  MLPCHK(hipMemsetAsync(m->d_loss, 0, sizeof(double), m->stream));
+  if (clear_loss) MLPCHK(hipMemsetAsync(m->d_loss, 0, sizeof(double), m->stream));
   hipLaunchKernelGGL(mse_kernel, dim3(64), dim3(256), 0, m->stream, m->d_act[L], m->d_t, n,
                      2.f / static_cast<float>(n), m->d_delta0, m->d_loss);
   MLPCHK(hipGetLastError());
@@ -531,16 +533,19 @@ extern "C" int scft_mlp_train(scft_mlp* m, int n_samples, const float* X, const 
       rc = SCFT_E_HIP;
       break;
     }
-    double esum = 0.0;
-    long long ecount = 0;
-    std::vector<double> bl;
+    // the epoch's squared error accumulates on the device and is read once
+    // at its end: the steps queue without a host round trip between them
+    if (hipMemsetAsync(m->d_loss, 0, sizeof(double), m->stream) != hipSuccess) {
+      rc = SCFT_E_HIP;
+      break;
+    }
     for (int b0 = 0; b0 < n_samples && rc == SCFT_OK; b0 += batch) {
       const int nb = std::min(batch, n_samples - b0);
       hipLaunchKernelGGL(gather_kernel, dim3(128), dim3(256), 0, m->stream, dX, dIdx + b0, nb,
                          din, m->d_act[0]);
       hipLaunchKernelGGL(gather_kernel, dim3(128), dim3(256), 0, m->stream, dY, dIdx + b0, nb,
                          dout, m->d_t);
-      if (hipGetLastError() != hipSuccess || forward(m, nb) || backward(m, nb)) {
+      if (hipGetLastError() != hipSuccess || forward(m, nb) || backward(m, nb, false)) {
         rc = SCFT_E_HIP;
         break;
       }
@@ -549,17 +554,17 @@ extern "C" int scft_mlp_train(scft_mlp* m, int n_samples, const float* X, const 
       const float c2 = 1.f - std::pow(b2, (float)m->adam_t);
       hipLaunchKernelGGL(adam_kernel, dim3(256), dim3(256), 0, m->stream, m->n_params, m->d_p,
                          m->d_g, m->d_m, m->d_v, lr, b1, b2, 1e-8f, c1, c2);
-      double l = 0.0;
-      if (hipMemcpyAsync(&l, m->d_loss, sizeof(double), hipMemcpyDeviceToHost, m->stream) !=
-              hipSuccess ||
-          hipStreamSynchronize(m->stream) != hipSuccess) {
-        rc = SCFT_E_HIP;
-        break;
-      }
-      esum += l;
-      ecount += (long long)nb * dout;
+      if (hipGetLastError() != hipSuccess) rc = SCFT_E_HIP;
     }
-    if (epoch_loss && rc == SCFT_OK) epoch_loss[e] = esum / static_cast<double>(ecount);
+    // also orders the next epoch's host shuffle after this epoch's index copy
+    double esum = 0.0;
+    if (rc == SCFT_OK &&
+        (hipMemcpyAsync(&esum, m->d_loss, sizeof(double), hipMemcpyDeviceToHost, m->stream) !=
+             hipSuccess ||
+         hipStreamSynchronize(m->stream) != hipSuccess))
+      rc = SCFT_E_HIP;
+    if (epoch_loss && rc == SCFT_OK)
+      epoch_loss[e] = esum / (static_cast<double>(n_samples) * dout);
   }
   (void)hipStreamSynchronize(m->stream);
   (void)hipFree(dX);

@@ -174,9 +174,16 @@ class WarmStart:
         self.S = S
         self.mlp = S.Mlp(DIMS, seed=seed)
 
-    def fit(self, X, Y, epochs=50, batch=256, lr=1e-3, seed=2):
+    def fit(self, X, Y, epochs=50, batch=256, lr=1e-3, seed=2, schedule=None):
+        """Adam on the mean squared error of the normalised targets: `epochs`
+        at `lr`, or the stages of `schedule` ((epochs, lr), ...) one after
+        another with the optimiser state carried over.  Returns the loss of
+        every epoch."""
         self.mlp.set_normalisation(*normalisation(X, Y))
-        return self.mlp.train(X, Y, epochs=epochs, batch=batch, lr=lr, seed=seed)
+        stages = schedule if schedule else ((epochs, lr),)
+        hist = [self.mlp.train(X, Y, epochs=int(e), batch=batch, lr=float(r), seed=seed + k)
+                for k, (e, r) in enumerate(stages)]
+        return np.concatenate(hist)
 
     def predict(self, cfg: dict) -> list:
         """Initial fields (all nodes) for every instance of a configuration."""
