@@ -100,10 +100,16 @@ WARM_FIELD_WEIGHT = 4.0
 # diverge in the warm form; an instance not converged after 24 evaluations
 # goes over to the cold form with its current field (history cleared).  On
 # the 16384-instance production mix that is ~3% of the instances, and the
-# batch time is lowest there (16: 1.37 s, 20: 1.41 s, 24: 1.33 s per solve;
-# without the switch ~1% never converge); on the 4096-instance mix no
-# instance needs it (the slowest converges in 21).
+# batch time is lowest there (16: 1.37 s, 20: 1.41 s, 24: 1.33 s per solve
+# with a 50-epoch network; without the switch ~1% never converge).  With the
+# staged network below, no instance of either mix needs it.
 WARM_SWITCH = 24
+# From predicted starts the mixer needs a shorter history than from w0: 8
+# columns take the same mean evaluations as 16 (production mix 8.34 / 8.39,
+# 4096-instance mix 6.64 / 6.64) with a cheaper Gram pass and a shorter tail
+# (0.99 s against 1.05 s, 93 ms against 103 ms per solve).  The cold leg
+# keeps 16, which its fine meshes need.
+WARM_HISTORY = 8
 # The network: 40 000 converged fields, Adam in four stages of 100 epochs
 # with the rate stepped down 1e-3 -> 3e-5.  A single 50-epoch stage at 1e-3
 # leaves networks whose warm solves vary with the shuffling seed (production
@@ -119,7 +125,8 @@ def solver_settings(workload):
 
 
 def solver_text(workload):
-    return (f"device Anderson, history 16, log mixing preconditioned by I + "
+    return (f"device Anderson, history 16 (cold) or {WARM_HISTORY} (predicted starts), log "
+            f"mixing preconditioned by I + "
             f"{WORKLOADS[workload]['precond']} M^-1 K (cold) or I + {WARM_PRECOND} "
             f"d^-1 (M^-1 K + {WARM_FIELD_WEIGHT} W+) d, d = sqrt(phi_t) (predicted starts, "
             f"cold form after {WARM_SWITCH} evaluations), "
@@ -204,6 +211,7 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     wopts.warm_precond = WARM_PRECOND
     wopts.warm_field_weight = WARM_FIELD_WEIGHT
     wopts.warm_switch = WARM_SWITCH
+    wopts.history = WARM_HISTORY
     w0_flat = b.join(w0)
     b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
     plan = ws.device_plan(b, prep)  # network inputs and mapping tables, resident in HBM
