@@ -271,8 +271,11 @@ def continuation_compare(args):
     back), the coarse batch is solved from w0, its fields are moved onto the
     bench meshes by the not-a-knot spline (scft_resample_spline_batch, as
     spline_chen.c), and the bench batch is solved from them with the cold
-    fallback.  Timed per step: both solves and the transfer.  The cold leg is
-    timed beside it; the MLP leg is the bench line's own on the same batch."""
+    fallback.  Timed per step: both solves and the transfer.  The fine solve
+    runs twice: with the cold form of the mixer, and with the warm-start
+    form and history the MLP leg uses (the transferred field is a predicted
+    start too).  The cold leg is timed beside it; the MLP leg is the bench
+    line's own on the same batch."""
     import scft_amd as S
     from scft_amd import workloads as WL
     if args.n_inst <= 0:
@@ -291,12 +294,18 @@ def continuation_compare(args):
     opts = solver_opts(S, args)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
+    wform = solver_opts(S, args)
+    wform.fallback_factor = 10.0
+    wform.warm_precond = WARM_PRECOND
+    wform.warm_field_weight = WARM_FIELD_WEIGHT
+    wform.warm_switch = WARM_SWITCH
+    wform.history = WARM_HISTORY
     w0_flat = fine.join(w0)
     # the spline transfer: coarse unknowns (with the wall node) -> fine unknowns
     src_x = [x for x in xc]
     dst_x = [x[1:] for x in xs]
 
-    def cont_step():
+    def cont_step(fine_opts):
         t0 = time.perf_counter()
         coarse.upload(w0c)
         rc = coarse.solve_resident(opts)
@@ -306,7 +315,7 @@ def continuation_compare(args):
         wt = S.resample_spline_batch(src_x, src_y, dst_x, flat=True)[0]
         fine.upload(wt, w0_flat)
         t2 = time.perf_counter()
-        rf = fine.solve_resident(wopts)
+        rf = fine.solve_resident(fine_opts)
         t3 = time.perf_counter()
         return t3 - t0, (t1 - t0, t2 - t1, rf["seconds"]), resc, rc
 
@@ -316,29 +325,36 @@ def continuation_compare(args):
         return r["seconds"]
 
     for _ in range(args.warmup):
-        cont_step()
+        cont_step(wopts)
+        cont_step(wform)
         cold_step()
-    tc, tw, parts = 0.0, 0.0, np.zeros(3)
+    tw = 0.0
+    tc, parts, res, resc = {}, {}, {}, None
     for i in range(args.steps):
         tw += cold_step()
         res_cold = fine.download()
-        dt, pp, resc, rc = cont_step()
-        res = fine.download()
-        tc += dt
-        parts += pp
-        log(f"bench: continuation step {i}: coarse solve {pp[0]:.3f} s ({rc['sweeps']} sweeps, "
-            f"mean iters {resc['iters'].mean():.1f}), transfer {pp[1]:.3f} s, fine solve "
-            f"{pp[2]:.3f} s ({res['sweeps']} sweeps, mean iters {res['iters'].mean():.1f}, "
-            f"max {res['iters'].max()}); cold {res_cold['iters'].mean():.1f} mean iters")
+        for form, fo in (("cold_form", wopts), ("warm_form", wform)):
+            dt, pp, resc, rc = cont_step(fo)
+            res[form] = fine.download()
+            tc[form] = tc.get(form, 0.0) + dt
+            parts[form] = parts.get(form, np.zeros(3)) + pp
+            r = res[form]
+            log(f"bench: continuation ({form}) step {i}: coarse solve {pp[0]:.3f} s "
+                f"({rc['sweeps']} sweeps, mean iters {resc['iters'].mean():.1f}), transfer "
+                f"{pp[1]:.3f} s, fine solve {pp[2]:.3f} s ({r['sweeps']} sweeps, mean iters "
+                f"{r['iters'].mean():.1f}, max {r['iters'].max()}); cold "
+                f"{res_cold['iters'].mean():.1f} mean iters")
     out = {"workload": WORKLOADS[args.workload]["text"].format(n=args.n_inst),
            "cold_solves_per_s": round(args.n_inst * args.steps / tw, 1),
-           "continuation_solves_per_s": round(args.n_inst * args.steps / tc, 1),
-           "continuation_seconds_coarse_transfer_fine": [round(v / args.steps, 4) for v in parts],
-           "continuation_fine_mean_iters": round(float(res["iters"].mean()), 2),
-           "continuation_fine_max_iters": int(res["iters"].max()),
-           "continuation_coarse_mean_iters": round(float(resc["iters"].mean()), 2),
            "cold_mean_iters": round(float(res_cold["iters"].mean()), 2),
-           "converged": int(res["converged"])}
+           "continuation_coarse_mean_iters": round(float(resc["iters"].mean()), 2)}
+    for form in ("cold_form", "warm_form"):
+        r = res[form]
+        out[f"continuation_{form}"] = {
+            "solves_per_s": round(args.n_inst * args.steps / tc[form], 1),
+            "seconds_coarse_transfer_fine": [round(v / args.steps, 4) for v in parts[form]],
+            "fine_mean_iters": round(float(r["iters"].mean()), 2),
+            "fine_max_iters": int(r["iters"].max()), "converged": int(r["converged"])}
     fine.close()
     coarse.close()
     print(json.dumps(out), flush=True)
