@@ -10,41 +10,75 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 
-def test_warm_start_reduces_iterations_on_held_out_instances():
+def _train_fields(S, WS, WL, opts, n_fields, seed0):
+    Xs, Ys, left, part = [], [], n_fields, 0
+    while left > 0:
+        n = min(8192, left)
+        cfg = WL.config3(n_inst=n, seed=seed0 + 101 * part)
+        out = WS.solve(S, cfg, opts)
+        conv = (out["status"] & S.ST_CONVERGED) != 0
+        assert conv.all()
+        X, Y = WS.dataset(S, cfg, out["w_nodes"], conv)
+        Xs.append(X)
+        Ys.append(Y)
+        left -= n
+        part += 1
+    return np.concatenate(Xs), np.concatenate(Ys)
+
+
+def test_warm_start_on_held_out_instances_as_specified():
+    """The surrogate evaluation as the workload specifies it: 20 000 fields
+    the batched solver converged (configuration-3 distribution), Adam at rate
+    1e-3, batch 256, 50 epochs; then 2048 instances from disjoint seeds solved
+    from w0 and from the prediction (device plan, outliers back to w0), with
+    the evaluation counts of both.  The prediction starts mix in the
+    warm-start form (history 8, plain form after 24 evaluations).  Both
+    solves converge every instance to the same solution.  Measured: mean 14.6
+    evaluations cold, 7.8 warm (training is deterministic for a seed)."""
     import scft_amd as S
     from scft_amd import warmstart as WS, workloads as WL
-    opts = S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0,
-                           transform=S.MIX_LOG, precond=0.5, backtrack=1e30, check_every=4)
-    train = WL.config3(n_inst=2048, seed=4242, Ns=(512, 1024))
-    out = WS.solve(S, train, opts)
-    conv = (out["status"] & S.ST_CONVERGED) != 0
-    assert conv.all()
-    X, Y = WS.dataset(S, train, out["w_nodes"], conv)
+    kw = dict(beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0, transform=S.MIX_LOG,
+              precond=0.5, backtrack=1e30, check_every=4, floor=1e-8, field_floor=0.5)
+    opts = S.anderson_opts(history=16, **kw)
+    X, Y = _train_fields(S, WS, WL, opts, 20000, 4242)
     ws = WS.WarmStart(S, seed=5)
-    hist = ws.fit(X, Y, epochs=100, batch=256, lr=1e-3, seed=6)
-    print(f"training loss {hist[0]:.3e} -> {hist[-1]:.3e}")
-    assert hist[-1] < 0.5 * hist[0]
+    hist = ws.fit(X, Y, epochs=50, batch=256, lr=1e-3, seed=6)
+    print(f"{len(X)} training fields, loss {hist[0]:.3e} -> {hist[-1]:.3e}", flush=True)
+    assert hist[-1] < 0.2 * hist[0]
 
-    test = WL.config3(n_inst=512, seed=99, Ns=(512, 1024))  # disjoint seeds
-    cold = WS.solve(S, test, opts)
-    wi = ws.predict(test)
-    wopts = S.anderson_opts(history=10, beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0,
-                            transform=S.MIX_LOG, precond=0.5, backtrack=1e30, check_every=4,
-                            fallback_factor=10.0)
-    warm = WS.solve(S, test, wopts, w_init=wi, w_cold=test["w0"])
+    test = WL.config3(n_inst=2048, seed=99)  # seeds disjoint from the training draws
+    w0 = [w[1:] for w in test["w0"]]
+    b = S.Batch(test["xs"], test["Ns"], [p[1:] for p in test["pts"]], bc=S.BC_DN)
+    b.upload(w0)
+    b.solve_resident(opts)
+    cold = b.download()
+    w0f = b.join(w0)
+    b.upload(w0f, w0f)
+    prep = WS.Prepared(S, test)
+    plan = ws.device_plan(b, prep)
+    plan.predict()
+    wopts = S.anderson_opts(history=8, fallback_factor=10.0, warm_precond=0.5,
+                            warm_field_weight=4.0, warm_switch=24, **kw)
+    b.solve_resident(wopts)
+    warm = b.download()
+    plan.close()
+    b.close()
+    ws.close()
     ic, iw = cold["iters"].mean(), warm["iters"].mean()
-    print(f"held-out mean evaluations: cold {ic:.2f}, warm {iw:.2f}; "
-          f"fallbacks {int(((warm['status'] & S.ST_FALLBACK) != 0).sum())}")
-    assert cold["converged"] == 512 and warm["converged"] == 512
-    assert iw <= 0.9 * ic
+    print(f"held-out mean evaluations: cold {ic:.2f} (max {cold['iters'].max()}), warm "
+          f"{iw:.2f} (max {warm['iters'].max()}); fallbacks "
+          f"{int(((warm['status'] & S.ST_FALLBACK) != 0).sum())}", flush=True)
+    assert cold["converged"] == 2048 and warm["converged"] == 2048
+    assert iw <= 0.6 * ic
     # same solutions: the oracle's evaluator on a sample of both results
-    for k in range(0, 512, 64):
+    for k in range(0, 2048, 256):
         x, N, pt = test["xs"][k], test["Ns"][k], test["pts"][k]
-        ea = O.evaluate(x, cold["w_nodes"][k], pt, N, bc=O.BC_DN)
-        eb = O.evaluate(x, warm["w_nodes"][k], pt, N, bc=O.BC_DN)
+        wa = np.concatenate([[0.0], cold["w"][k]])
+        wb = np.concatenate([[0.0], warm["w"][k]])
+        ea = O.evaluate(x, wa, pt, N, bc=O.BC_DN)
+        eb = O.evaluate(x, wb, pt, N, bc=O.BC_DN)
         assert ea["rnorm"] <= 1e-6 + 1e-10 and eb["rnorm"] <= 1e-6 + 1e-10
         assert np.abs(ea["phi"] - eb["phi"]).max() <= 3e-6
-    ws.close()
 
 
 def test_device_warm_plan_matches_host_mapping():
