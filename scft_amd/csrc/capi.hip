@@ -315,8 +315,8 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
     const int M = dsc->M[k], N = dsc->N[k];
     // N: a power of two (Romberg) in [16, 16384] (the N/2 + 1 pairing weights
     // of a bucket are held in LDS)
-    // IRK4 keeps more per-row state in registers: up to 2048 unknowns
-    const int n_max = (b->stepper == SCFT_STEP_IRK4) ? 2048 : 4096;
+    // up to 4096 unknowns (both steppers)
+    const int n_max = 4096;
     if (M < 3 || (b->bc == SCFT_BC_DD ? M - 2 : M - 1) > n_max || N < 16 || N > 16384 ||
         (N & (N - 1)) != 0) {
       snprintf(g_errmsg, sizeof g_errmsg, "instance %d: M=%d N=%d unsupported", k, M, N);

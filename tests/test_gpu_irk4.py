@@ -40,6 +40,11 @@ def _cases():
         c.append((f"edge_m{m}", m, 128, 2.0 + m % 7, bc, O.C_ROWSCALED, None, None))
     c.append(("max_m2048_DN", 2048, 256, 8.0, O.BC_DN, O.C_ROWSCALED, None, None))
     c.append(("max_m2049_DD", 2049, 128, 8.0, O.BC_DD, O.C_ROWSCALED, None, None))
+    # the 4096-unknown bucket (the spilling (8, 8) variant)
+    c.append(("m2049_DN", 2049, 64, 8.0, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("m3001_DD", 3001, 128, 8.0, O.BC_DD, O.C_ROWSCALED, None, None))
+    c.append(("max_m4096_DN", 4096, 64, 8.0, O.BC_DN, O.C_ROWSCALED, None, None))
+    c.append(("max_m4097_DD", 4097, 64, 8.0, O.BC_DD, O.C_CONSISTENT, None, None))
     c.append(("consistent_m100_DD", 100, 256, 3.0, O.BC_DD, O.C_CONSISTENT, None, None))
     c.append(("consistent_m700_DN", 700, 512, 4.0, O.BC_DN, O.C_CONSISTENT, None, None))
     c.append(("graded_DN", None, 1024, 5.0, O.BC_DN, O.C_ROWSCALED, (1e-3, 0.2), None))
@@ -86,10 +91,11 @@ def test_irk4_march_matches_oracle(case):
 
 
 def test_irk4_mixed_batch_matches_oracle():
-    """Every IRK4 bucket (64 .. 2048 unknowns) populated in one sweep."""
+    """Every IRK4 bucket (64 .. 4096 unknowns) populated in one sweep."""
     S = _scft()
     rng = np.random.default_rng(4242)
-    ms = [32, 64, 65, 127, 128, 200, 256, 300, 511, 512, 700, 1023, 1024, 1500, 2047, 2048]
+    ms = [32, 64, 65, 127, 128, 200, 256, 300, 511, 512, 700, 1023, 1024, 1500, 2047, 2048,
+          2049, 3500, 4096]
     Ns = [int(rng.choice([64, 128, 256])) for _ in ms]
     Ls = rng.uniform(2.0, 8.0, len(ms))
     insts = [instance(int(m), float(L), N, bc=O.BC_DN, seed=300 + i)
@@ -143,12 +149,12 @@ def test_irk4_dealii_fixture_batch_and_dropin():
     S.finalize()
 
 
-def test_irk4_rejects_more_than_2048_unknowns():
+def test_irk4_rejects_more_than_4096_unknowns():
     S = _scft()
-    x = O.mesh_uniform(2050, 4.0)
+    x = O.mesh_uniform(4098, 4.0)
     with pytest.raises(S.ScftError):
-        S.Batch([x], [64], [np.ones(2049)], bc=S.BC_DN, stepper=S.STEP_IRK4)
-    b = S.Batch([x], [64], [np.ones(2048)], bc=S.BC_DD, stepper=S.STEP_IRK4)  # n = 2048
+        S.Batch([x], [64], [np.ones(4097)], bc=S.BC_DN, stepper=S.STEP_IRK4)
+    b = S.Batch([x], [64], [np.ones(4096)], bc=S.BC_DD, stepper=S.STEP_IRK4)  # n = 4096
     b.close()
 
 
