@@ -87,28 +87,24 @@ WORKLOADS = {
 }
 
 
-# The warm-start form of the preconditioner used from predicted starts:
-# g = f + 0.5 d^-1 (M_L^-1 K + 4 diag(max(w, 0))) (d f), d = sqrt(phi_t).
-# From exact converged fields passed through the network's output
-# representation (the best a prediction can be) it needs 7.1 evaluations on
-# average on the production mix and 5.7 on configuration 3, against 16.6 and
-# 11.2 with the cold form (CPU oracle, 24 instances each); from a cold start
-# it does not converge, so the cold leg keeps the cold form.
-WARM_PRECOND = 0.5
-WARM_FIELD_WEIGHT = 4.0
-# Some predicted starts (stiff graded meshes with N = 512 mostly) stall or
-# diverge in the warm form; an instance not converged after 24 evaluations
-# goes over to the cold form with its current field (history cleared).  On
-# the 16384-instance production mix that is ~3% of the instances, and the
-# batch time is lowest there (16: 1.37 s, 20: 1.41 s, 24: 1.33 s per solve
-# with a 50-epoch network; without the switch ~1% never converge).  With the
-# staged network below, no instance of either mix needs it.
-WARM_SWITCH = 24
-# From predicted starts the mixer needs a shorter history than from w0: 8
-# columns take the same mean evaluations as 16 (production mix 8.34 / 8.39,
-# 4096-instance mix 6.64 / 6.64) with a cheaper Gram pass and a shorter tail
-# (0.99 s against 1.05 s, 93 ms against 103 ms per solve).  The cold leg
-# keeps 16, which its fine meshes need.
+# The ground-state form of the preconditioner, used by every leg:
+# g_u = f_u + 0.5 [ l (d_{u-1}/d_u)(f_u - f_{u-1}) + r (d_{u+1}/d_u)(f_u - f_{u+1})
+#                   + 1.0 max(w_u, 0) f_u ],  d = sqrt(phi_t),
+# the operator -(1/d^2)(d^2 f')' + w+.  On production-mix instances the
+# preconditioned Jacobian has every eigenvalue within 0.14 of 1 with it.
+# CPU oracle, evaluations (mean / max): from the network's predictions 3.4 / 5
+# on 96 random instances and 4.0 / 6 on 48 stiff ones (dt/h0^2 >= 1.4e4),
+# against 8.4 / 15 for the earlier d^-1 (M_L^-1 K + 4 W+) d form; from w0
+# 13.7 / 21 against 20.1 / 27 for the plain form I + 0.6 M_L^-1 K (48
+# random) and 22.0 / 32 against 26.8 / 38 (48 stiff).  A field weight of 4
+# instead of 1 is as good from predictions and fails from w0.
+GS_PRECOND = 0.5
+GS_FIELD_WEIGHT = 1.0
+# Predicted starts: an instance not converged after 24 evaluations goes over
+# to the plain form with its current field (history cleared), a safety net
+# that no instance of either bench mix reaches.
+GS_SWITCH = 24
+# Predicted starts need a shorter history than w0: 8 columns.
 WARM_HISTORY = 8
 # The network: 40 000 converged fields, Adam in four stages of 100 epochs
 # with the rate stepped down 1e-3 -> 3e-5.  A single 50-epoch stage at 1e-3
@@ -121,16 +117,16 @@ TRAIN_SCHEDULE = ((100, 1e-3), (100, 3e-4), (100, 1e-4), (100, 3e-5))
 def solver_settings(workload):
     return dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8,
                 precond=WORKLOADS[workload]["precond"], backtrack=1e30, max_halvings=30,
-                field_floor=0.5)
+                field_floor=0.5, gs_precond=GS_PRECOND, gs_field_weight=GS_FIELD_WEIGHT)
 
 
 def solver_text(workload):
     return (f"device Anderson, history 16 (cold) or {WARM_HISTORY} (predicted starts), log "
-            f"mixing preconditioned by I + "
-            f"{WORKLOADS[workload]['precond']} M^-1 K (cold) or I + {WARM_PRECOND} "
-            f"d^-1 (M^-1 K + {WARM_FIELD_WEIGHT} W+) d, d = sqrt(phi_t) (predicted starts, "
-            f"cold form after {WARM_SWITCH} evaluations), "
-            "beta 1.0, no step cap, NaN backtracking, field bounded below by -0.5/dt")
+            f"mixing preconditioned by the ground-state form I + {GS_PRECOND} "
+            f"(-(1/d^2)(d^2 f')' + {GS_FIELD_WEIGHT} w+), d = sqrt(phi_t) (predicted starts: "
+            f"plain form I + {WORKLOADS[workload]['precond']} M^-1 K after {GS_SWITCH} "
+            "evaluations), beta 1.0, no step cap, NaN backtracking, field bounded below by "
+            "-0.5/dt")
 
 
 def make_workload(WL, workload, n_inst, seed=None):
@@ -205,12 +201,9 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     prep = WS.Prepared(S, cfg)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
-    # from a predicted start: the warm-start form of the preconditioner
-    # (q-weighted Laplacian screened by the local field); instances that fall
-    # back to w0 mix with the cold form
-    wopts.warm_precond = WARM_PRECOND
-    wopts.warm_field_weight = WARM_FIELD_WEIGHT
-    wopts.warm_switch = WARM_SWITCH
+    # from a predicted start: the ground-state form with its switch to the
+    # plain form, and a shorter history
+    wopts.gs_switch = GS_SWITCH
     wopts.history = WARM_HISTORY
     w0_flat = b.join(w0)
     b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
@@ -272,9 +265,8 @@ def continuation_compare(args):
     bench meshes by the not-a-knot spline (scft_resample_spline_batch, as
     spline_chen.c), and the bench batch is solved from them with the cold
     fallback.  Timed per step: both solves and the transfer.  The fine solve
-    runs twice: with the cold form of the mixer, and with the warm-start
-    form and history the MLP leg uses (the transferred field is a predicted
-    start too).  The cold leg is timed beside it; the MLP leg is the bench
+    uses the MLP leg's settings for predicted starts (the transferred field
+    is one).  The cold leg is timed beside it; the MLP leg is the bench
     line's own on the same batch."""
     import scft_amd as S
     from scft_amd import workloads as WL
@@ -294,18 +286,14 @@ def continuation_compare(args):
     opts = solver_opts(S, args)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
-    wform = solver_opts(S, args)
-    wform.fallback_factor = 10.0
-    wform.warm_precond = WARM_PRECOND
-    wform.warm_field_weight = WARM_FIELD_WEIGHT
-    wform.warm_switch = WARM_SWITCH
-    wform.history = WARM_HISTORY
+    wopts.gs_switch = GS_SWITCH
+    wopts.history = WARM_HISTORY
     w0_flat = fine.join(w0)
     # the spline transfer: coarse unknowns (with the wall node) -> fine unknowns
     src_x = [x for x in xc]
     dst_x = [x[1:] for x in xs]
 
-    def cont_step(fine_opts):
+    def cont_step():
         t0 = time.perf_counter()
         coarse.upload(w0c)
         rc = coarse.solve_resident(opts)
@@ -315,7 +303,7 @@ def continuation_compare(args):
         wt = S.resample_spline_batch(src_x, src_y, dst_x, flat=True)[0]
         fine.upload(wt, w0_flat)
         t2 = time.perf_counter()
-        rf = fine.solve_resident(fine_opts)
+        rf = fine.solve_resident(wopts)
         t3 = time.perf_counter()
         return t3 - t0, (t1 - t0, t2 - t1, rf["seconds"]), resc, rc
 
@@ -325,36 +313,29 @@ def continuation_compare(args):
         return r["seconds"]
 
     for _ in range(args.warmup):
-        cont_step(wopts)
-        cont_step(wform)
+        cont_step()
         cold_step()
-    tw = 0.0
-    tc, parts, res, resc = {}, {}, {}, None
+    tc, tw, parts = 0.0, 0.0, np.zeros(3)
     for i in range(args.steps):
         tw += cold_step()
         res_cold = fine.download()
-        for form, fo in (("cold_form", wopts), ("warm_form", wform)):
-            dt, pp, resc, rc = cont_step(fo)
-            res[form] = fine.download()
-            tc[form] = tc.get(form, 0.0) + dt
-            parts[form] = parts.get(form, np.zeros(3)) + pp
-            r = res[form]
-            log(f"bench: continuation ({form}) step {i}: coarse solve {pp[0]:.3f} s "
-                f"({rc['sweeps']} sweeps, mean iters {resc['iters'].mean():.1f}), transfer "
-                f"{pp[1]:.3f} s, fine solve {pp[2]:.3f} s ({r['sweeps']} sweeps, mean iters "
-                f"{r['iters'].mean():.1f}, max {r['iters'].max()}); cold "
-                f"{res_cold['iters'].mean():.1f} mean iters")
+        dt, pp, resc, rc = cont_step()
+        res = fine.download()
+        tc += dt
+        parts += pp
+        log(f"bench: continuation step {i}: coarse solve {pp[0]:.3f} s ({rc['sweeps']} sweeps, "
+            f"mean iters {resc['iters'].mean():.1f}), transfer {pp[1]:.3f} s, fine solve "
+            f"{pp[2]:.3f} s ({res['sweeps']} sweeps, mean iters {res['iters'].mean():.1f}, "
+            f"max {res['iters'].max()}); cold {res_cold['iters'].mean():.1f} mean iters")
     out = {"workload": WORKLOADS[args.workload]["text"].format(n=args.n_inst),
            "cold_solves_per_s": round(args.n_inst * args.steps / tw, 1),
+           "continuation_solves_per_s": round(args.n_inst * args.steps / tc, 1),
+           "continuation_seconds_coarse_transfer_fine": [round(v / args.steps, 4) for v in parts],
+           "continuation_fine_mean_iters": round(float(res["iters"].mean()), 2),
+           "continuation_fine_max_iters": int(res["iters"].max()),
+           "continuation_coarse_mean_iters": round(float(resc["iters"].mean()), 2),
            "cold_mean_iters": round(float(res_cold["iters"].mean()), 2),
-           "continuation_coarse_mean_iters": round(float(resc["iters"].mean()), 2)}
-    for form in ("cold_form", "warm_form"):
-        r = res[form]
-        out[f"continuation_{form}"] = {
-            "solves_per_s": round(args.n_inst * args.steps / tc[form], 1),
-            "seconds_coarse_transfer_fine": [round(v / args.steps, 4) for v in parts[form]],
-            "fine_mean_iters": round(float(r["iters"].mean()), 2),
-            "fine_max_iters": int(r["iters"].max()), "converged": int(r["converged"])}
+           "converged": int(res["converged"])}
     fine.close()
     coarse.close()
     print(json.dumps(out), flush=True)

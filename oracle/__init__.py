@@ -61,8 +61,9 @@ class AndersonOpts(C.Structure):
                 ("transform", C.c_int), ("floor", C.c_double), ("precond", C.c_double),
                 ("pstencil", C.c_void_p), ("backtrack", C.c_double), ("max_halvings", C.c_int),
                 ("field_floor", C.c_double), ("field_lower", C.c_double),
-                ("warm_precond", C.c_double), ("warm_field_weight", C.c_double),
-                ("warm_switch", C.c_int), ("warm_switch_growth", C.c_double)]
+                ("gs_precond", C.c_double), ("gs_field_weight", C.c_double),
+                ("gs_switch", C.c_int), ("gs_switch_growth", C.c_double),
+                ("gstencil", C.c_void_p)]
 
 
 class AndersonStats(C.Structure):
@@ -126,6 +127,7 @@ def lib():
         L.orc_batch_solve.argtypes = [C.c_int, _ip, _ip, _lp, _dp, _dp, _dp, C.c_int, C.c_int,
                                       C.POINTER(AndersonOpts), C.c_int, _ip, _ip, _dp]
         L.orc_precond_stencil.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.orc_gs_stencil.argtypes = [_dp, C.c_int, C.c_int, _dp]
         L.orc_batch_eval.argtypes = [C.c_int, _ip, _ip, _lp, _dp, _dp, _dp, C.c_int, C.c_int,
                                      C.c_int, _dp, _dp, _dp]
         L.orc_global_setup.argtypes = [C.c_void_p, C.c_void_p, C.c_long, C.c_long]
@@ -318,11 +320,11 @@ MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=2000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
                   precond=0.0, backtrack=0.0, max_halvings=30, field_floor=0.0, field_lower=0.0,
-                  warm_precond=0.0, warm_field_weight=4.0, warm_switch=0, warm_switch_growth=0.0):
+                  gs_precond=0.0, gs_field_weight=1.0, gs_switch=0, gs_switch_growth=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, precond, None, backtrack, max_halvings, field_floor,
-                        field_lower, warm_precond, warm_field_weight, warm_switch,
-                        warm_switch_growth)
+                        field_lower, gs_precond, gs_field_weight, gs_switch,
+                        gs_switch_growth, None)
 
 
 def solve_anderson(x, phi_t, N, w_full, opts=None, bc=BC_DN, cmat=C_ROWSCALED, trace_len=0):

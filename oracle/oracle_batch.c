@@ -68,15 +68,22 @@ int orc_solve_anderson(const orc_problem *p, double *w_full,
   memcpy(c.wfull, w_full, sizeof(double) * p->M);
   orc_anderson_opts oo = *o;
   if (oo.field_floor > 0.0) oo.field_lower = -oo.field_floor * p->N;
-  double *ps = NULL;
-  if ((oo.precond > 0.0 || oo.warm_precond > 0.0) && !oo.pstencil) { /* this mesh's stencil */
+  double *ps = NULL, *gs = NULL;
+  if (oo.precond > 0.0 && !oo.pstencil) { /* this mesh's stencils */
     ps = (double *)malloc(sizeof(double) * 3 * (size_t)n);
     if (!ps) { orc_ctx_free(&c); return ORC_E_ALLOC; }
     orc_precond_stencil(p->x, p->M, p->bc, ps);
     oo.pstencil = ps;
   }
+  if (oo.gs_precond > 0.0 && !oo.gstencil) {
+    gs = (double *)malloc(sizeof(double) * 3 * (size_t)n);
+    if (!gs) { free(ps); orc_ctx_free(&c); return ORC_E_ALLOC; }
+    orc_gs_stencil(p->x, p->M, p->bc, gs);
+    oo.gstencil = gs;
+  }
   int rc = orc_anderson(w_full + 1, n, orc_residual_cb, &c, &oo, p->phi_t + 1, st);
   free(ps);
+  free(gs);
   orc_ctx_free(&c);
   return rc;
 }

@@ -177,10 +177,26 @@ int orc_precond_stencil(const double *x, int M, int bc, double *st) {
   return ORC_OK;
 }
 
+int orc_gs_stencil(const double *x, int M, int bc, double *st) {
+  int n = orc_n_unknowns(M, bc);
+  if (n < 1) return ORC_E_ARG;
+  for (int u = 0; u < n; u++) {
+    int i = u + 1; /* node of unknown u; node i - 1 always exists */
+    double hl = x[i] - x[i - 1];
+    double hr = (i + 1 < M) ? x[i + 1] - x[i] : 0.0;
+    double m = 0.5 * (hl + hr);
+    st[3 * u + 0] = (1.0 / hl) / m;
+    st[3 * u + 1] = 0.0;
+    st[3 * u + 2] = (i + 1 < M) ? (1.0 / hr) / m : 0.0;
+  }
+  return ORC_OK;
+}
+
 int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
                  const orc_anderson_opts *o, const double *phi_t_unk,
                  orc_anderson_stats *st) {
-  if ((o->precond > 0.0 || o->warm_precond > 0.0) && !o->pstencil) return 3;
+  if (o->precond > 0.0 && !o->pstencil) return 3;
+  if (o->gs_precond > 0.0 && (!o->gstencil || !phi_t_unk)) return 3;
   int mh = o->history;
   int S = mh + 1;
   double *wh = (double *)calloc((size_t)n * S, sizeof(double));
@@ -191,7 +207,7 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
   double U[64 * 64], bb[64];
   int stored = 0, head = 0, k, rc = 1, restarts = 0, halvings = 0;
   double best = INFINITY, err = INFINITY, eacc = INFINITY;
-  int warm_on = o->warm_precond > 0.0;
+  int gs_on = o->gs_precond > 0.0;
   for (k = 0; k < o->max_iter; k++) {
     int ferr = F(n, w, r, ctx);
     err = 0.0;
@@ -219,9 +235,9 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
       stored = 0;
       restarts++;
     }
-    if (warm_on && ((o->warm_switch > 0 && k + 1 > o->warm_switch) ||
-                    (o->warm_switch_growth > 0.0 && err > o->warm_switch_growth * best))) {
-      warm_on = 0; /* leave the warm-start form: plain form, history cleared */
+    if (gs_on && ((o->gs_switch > 0 && k + 1 > o->gs_switch) ||
+                    (o->gs_switch_growth > 0.0 && err > o->gs_switch_growth * best))) {
+      gs_on = 0; /* leave the ground-state form: plain form, history cleared */
       stored = 0;
       restarts++;
     }
@@ -234,20 +250,23 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
         else r[i] = log(fmax(pt + r[i], o->floor * 1e-3) / pt);
       }
     }
-    /* preconditioned mixing residual g = f + a M_L^{-1} K f, or its
-     * warm-start form g = f + a_w d^-1 (M_L^{-1} K + c W+) (d f) */
-    if (warm_on && phi_t_unk) {
-      const double *ps = o->pstencil;
-      double *h = (double *)malloc(sizeof(double) * n);
-      for (int i = 0; i < n; i++) h[i] = sqrt(fmax(phi_t_unk[i], o->floor)) * r[i];
+    /* preconditioned mixing residual g = f + a M_L^{-1} K f, or the
+     * ground-state form for starts near the solution:
+     *   g_u = f_u + a_w [ l_u (d_{u-1}/d_u)(f_u - f_{u-1})
+     *                   + r_u (d_{u+1}/d_u)(f_u - f_{u+1}) + c max(w_u,0) f_u ]
+     * with d = sqrt(max(phi_t, floor)) and d = 0 beyond the unknowns (the
+     * wall nodes; a natural end has r_u = 0) */
+    if (gs_on) {
+      const double *gs = o->gstencil;
       for (int i = 0; i < n; i++) {
-        const double hl = i > 0 ? h[i - 1] : 0.0, hr = (i + 1 < n) ? h[i + 1] : 0.0;
-        double kh = ps[3 * i + 1] * h[i] - ps[3 * i] * hl - ps[3 * i + 2] * hr;
-        kh = (kh + o->warm_field_weight * fmax(w[i], 0.0) * h[i]) /
-             sqrt(fmax(phi_t_unk[i], o->floor));
-        wn[i] = r[i] + o->warm_precond * kh;
+        const double di = sqrt(fmax(phi_t_unk[i], o->floor));
+        const double dl = i > 0 ? sqrt(fmax(phi_t_unk[i - 1], o->floor)) : 0.0;
+        const double dr = (i + 1 < n) ? sqrt(fmax(phi_t_unk[i + 1], o->floor)) : 0.0;
+        const double fl = i > 0 ? r[i - 1] : 0.0, fr = (i + 1 < n) ? r[i + 1] : 0.0;
+        double t = gs[3 * i] * (dl / di) * (r[i] - fl) + gs[3 * i + 2] * (dr / di) * (r[i] - fr);
+        t += o->gs_field_weight * fmax(w[i], 0.0) * r[i];
+        wn[i] = r[i] + o->gs_precond * t;
       }
-      free(h);
       memcpy(r, wn, sizeof(double) * n);
     } else if (o->precond > 0.0) {
       const double *ps = o->pstencil;

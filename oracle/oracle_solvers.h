@@ -45,13 +45,19 @@ typedef struct orc_anderson_opts {
   double field_lower;   /* the bound itself, -field_floor * N: set from the
                            instance's N by orc_solve_anderson (a caller of
                            orc_anderson sets it directly) */
-  double warm_precond;  /* a_w > 0: g = f + a_w d^-1 (M_L^{-1} K + c diag(max(w,0)))
-                           (d f), d = sqrt(max(phi_t, floor)), instead of the
-                           plain form (needs pstencil and phi_t_unk) */
-  double warm_field_weight; /* c */
-  int warm_switch;      /* leave the warm form (history cleared) after this
+  double gs_precond;  /* a_w > 0: the ground-state form
+                           g_u = f_u + a_w [ l_u (d_{u-1}/d_u)(f_u - f_{u-1})
+                                           + r_u (d_{u+1}/d_u)(f_u - f_{u+1})
+                                           + c max(w_u, 0) f_u ],
+                           d = sqrt(max(phi_t, floor)), d = 0 on a wall node,
+                           l, r from gstencil, instead of the plain form (needs
+                           gstencil and phi_t_unk) */
+  double gs_field_weight; /* c */
+  int gs_switch;      /* leave the ground-state form (history cleared) after this
                            many evaluations without converging (0: never) */
-  double warm_switch_growth; /* ... or when err > growth * best (0: never) */
+  double gs_switch_growth; /* ... or when err > growth * best (0: never) */
+  const double *gstencil; /* 3 n doubles (l, unused, r per unknown) of the
+                             ground-state form when gs_precond > 0 */
 } orc_anderson_opts;
 
 /* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements
@@ -60,6 +66,14 @@ typedef struct orc_anderson_opts {
  * (M_L^{-1} K f)_u = d_u f_u - l_u f_{u-1} - r_u f_{u+1}, d_u = l_u + r_u.
  * st receives 3 n doubles. */
 int orc_precond_stencil(const double *x, int M, int bc, double *st);
+
+/* Coefficients of the ground-state form's Laplacian, -(1/d^2) (d^2 f')',
+ * on the nodes of the mesh: unknown u (node i = u + 1) couples to nodes
+ * i - 1 and i + 1 with l_u = (1/h_l)/m_u and r_u = (1/h_r)/m_u, m_u = (h_l +
+ * h_r)/2, where the wall nodes count as neighbours (their d is zero, so
+ * their terms vanish) and a natural end (DN at x = L) has h_r = 0, r_u = 0.
+ * st receives 3 n doubles (l, 0, r). */
+int orc_gs_stencil(const double *x, int M, int bc, double *st);
 
 enum { ORC_MIX_PLAIN = 0, ORC_MIX_RELATIVE = 1, ORC_MIX_LOG = 2 };
 

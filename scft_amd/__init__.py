@@ -28,7 +28,7 @@ BC_DD, BC_DN = 0, 1
 C_ROWSCALED, C_CONSISTENT = 0, 1
 STEP_IE, STEP_IRK4 = 0, 1
 ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK, ST_FAILED = 1, 2, 4, 8, 16, 32
-ST_COLD_FORM = 64  # left the warm-start preconditioner
+ST_PLAIN_FORM = 64  # left the ground-state form of the preconditioner
 MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 L_REF = 3.72374357332160
 TAU = 0.530252230020752
@@ -65,9 +65,9 @@ class AndersonOpts(C.Structure):
                 ("fallback_factor", C.c_double), ("check_every", C.c_int),
                 ("progress_seconds", C.c_double), ("precond", C.c_double),
                 ("backtrack", C.c_double), ("max_halvings", C.c_int),
-                ("field_floor", C.c_double), ("warm_precond", C.c_double),
-                ("warm_field_weight", C.c_double), ("warm_switch", C.c_int),
-                ("warm_switch_growth", C.c_double)]
+                ("field_floor", C.c_double), ("gs_precond", C.c_double),
+                ("gs_field_weight", C.c_double), ("gs_switch", C.c_int),
+                ("gs_switch_growth", C.c_double)]
 
 
 class BroydenStats(C.Structure):
@@ -218,12 +218,12 @@ def local_iteration() -> int:
 def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
                   fallback_factor=0.0, check_every=8, progress_seconds=0.0, precond=0.0,
-                  backtrack=0.0, max_halvings=30, field_floor=0.0, warm_precond=0.0,
-                  warm_field_weight=4.0, warm_switch=0, warm_switch_growth=0.0):
+                  backtrack=0.0, max_halvings=30, field_floor=0.0, gs_precond=0.0,
+                  gs_field_weight=1.0, gs_switch=0, gs_switch_growth=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, fallback_factor, check_every, progress_seconds,
-                        precond, backtrack, max_halvings, field_floor, warm_precond,
-                        warm_field_weight, warm_switch, warm_switch_growth)
+                        precond, backtrack, max_halvings, field_floor, gs_precond,
+                        gs_field_weight, gs_switch, gs_switch_growth)
 
 
 class Batch:

@@ -23,8 +23,15 @@
 //  * backtracking: a failed evaluation (NaN / zero pivot) or one whose ||r||
 //    exceeds backtrack times the last accepted one halves the step toward the
 //    last accepted iterate (ring slot head - 1) and clears the history;
-//  * a warm-start preconditioner for instances started near the solution
-//    (below, at the push of the history);
+//  * the ground-state form of the preconditioner (below, at the push of the
+//    history): the Laplacian conjugated by d = sqrt(phi_t), the propagator's
+//    profile next to the wall, with the potential that makes constants its
+//    null space, -(1/d^2)(d^2 f')'.  In the log transform the Jacobian of f
+//    near the solution is ~ -d^-1 L^-1 d with L = -d_xx + w, so the form
+//    inverts it where the plain Laplacian does not: the wall layer, where
+//    phi_t falls to zero.  The preconditioned Jacobian of production-mix
+//    instances has every eigenvalue within 0.14 of 1 with it (the plain
+//    form: ~half of them beyond 0.2);
 //  * a lower bound w >= -field_floor / dt on the updated field.  Near a wall
 //    on a fine mesh the converged field spans decades (w at the first node
 //    1e6-1e8 when dt/h^2 ~ 1e4-1e5), and the preconditioned step from a smooth
@@ -149,18 +156,17 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     stored = 0;
     ms.restarts++;
   }
-  // the warm-start form of the preconditioner (below) while the instance
-  // started from a prediction and neither fell back to the cold field nor
-  // left the form: it leaves it (history cleared, status COLD_FORM) when it
-  // has used warm_switch evaluations or its error grows past
-  // warm_switch_growth times the best so far
-  bool wp = a.warm_precond > 0.0 && !(st0 & (ST_FALLBACK | ST_COLD_FORM));
-  if (wp && ((a.warm_switch > 0 && k > a.warm_switch) ||
-             (a.warm_switch_growth > 0.0 && err > a.warm_switch_growth * ms.best))) {
-    wp = false;
+  // the ground-state form of the preconditioner (below) until the instance
+  // leaves it for the plain form (history cleared, status PLAIN_FORM): after
+  // gs_switch evaluations, or when its error grows past gs_switch_growth
+  // times the best so far
+  bool gs = a.gs_precond > 0.0 && !(st0 & ST_PLAIN_FORM);
+  if (gs && ((a.gs_switch > 0 && k > a.gs_switch) ||
+             (a.gs_switch_growth > 0.0 && err > a.gs_switch_growth * ms.best))) {
+    gs = false;
     stored = 0;
     ms.restarts++;
-    if (tid == 0) a.status[inst] = st0 | ST_COLD_FORM;
+    if (tid == 0) a.status[inst] = st0 | ST_PLAIN_FORM;
   }
   if (err < ms.best) ms.best = err;
   const int cur = __builtin_amdgcn_readfirstlane(ms.head);
@@ -168,35 +174,58 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
 
   // push (w_k, g_k) into slot `cur`
   const double* __restrict__ xn = a.x + d.xo;
-  // the warm-start form of the preconditioner, for instances started from a
-  // prediction that did not fall back to the cold field: the stencil acts
-  // on q-weighted residuals h = d f, d = sqrt(max(phi_t, floor)) ~ q near
-  // the wall, plus the screening of the local field c max(w, 0):
-  //   g = f + a_w d^-1 (M_L^-1 K + c diag(max(w, 0))) (d f)
-  // (with d = 1 and c = 0 it is the plain form, bit for bit)
-  const double coef = wp ? a.warm_precond : a.precond;
-  const bool pc = coef > 0.0 && n > 1;
-  // (whole-wave trips: the neighbours' mixing residuals come from the lanes
-  // beside, and only a wave's edge lanes evaluate theirs again)
+  // the preconditioned mixing residual g: the plain form
+  //   g = f + a M_L^-1 K f   (lumped-mass Laplacian of the unknowns' grid,
+  //                           natural rows at both ends)
+  // or the ground-state form
+  //   g_u = f_u + a_g [ l_u (d_{u-1}/d_u)(f_u - f_{u-1})
+  //                   + r_u (d_{u+1}/d_u)(f_u - f_{u+1}) + c max(w_u, 0) f_u ]
+  // with d = sqrt(max(phi_t, floor)), zero beyond the unknowns (the wall
+  // nodes), l_u = (1/h_l)/m_u, r_u = (1/h_r)/m_u, m_u = (h_l + h_r)/2 over
+  // the mesh's nodes (a natural end has no right neighbour): the operator
+  // -(1/d^2)(d^2 f')' + c w+, which maps constants to themselves
+  const double coef = gs ? a.gs_precond : a.precond;
+  const bool pc = coef > 0.0 && (gs || n > 1);
+  // (whole-wave trips: the neighbours' values come from the lanes beside,
+  // and only a wave's edge lanes evaluate theirs again)
   for (int u0 = 0; u0 < n; u0 += AM_THREADS) {
     const int u = u0 + tid, lane = tid & 63;
     const bool act = u < n;
     double f = act ? mix_residual(r[u], pt[u], a.transform, a.floor) : 0.0;
-    if (pc) {
-      const double dq = (wp && act) ? sqrt(fmax(pt[u], a.floor)) : 1.0;
-      const double hv = dq * f;
-      double fl = __shfl_up(hv, 1), fr = __shfl_down(hv, 1);
+    if (pc && gs) {
+      const double dv = act ? sqrt(fmax(pt[u], a.floor)) : 1.0;
+      double fl = __shfl_up(f, 1), fr = __shfl_down(f, 1);
+      double dl = __shfl_up(dv, 1), dr = __shfl_down(dv, 1);
       if (act) {
-        const int i = u + 1;  // node of unknown u
+        const int i = u + 1;  // node of unknown u; node i - 1 always exists
         const bool has_l = u > 0, has_r = u + 1 < n;
         if (lane == 0 && has_l) {
           fl = mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor);
-          if (wp) fl = sqrt(fmax(pt[u - 1], a.floor)) * fl;
+          dl = sqrt(fmax(pt[u - 1], a.floor));
         }
         if (lane == 63 && has_r) {
           fr = mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor);
-          if (wp) fr = sqrt(fmax(pt[u + 1], a.floor)) * fr;
+          dr = sqrt(fmax(pt[u + 1], a.floor));
         }
+        if (!has_l) fl = dl = 0.0;
+        if (!has_r) fr = dr = 0.0;
+        const bool node_r = i + 1 < d.M;
+        const double hl = xn[i] - xn[i - 1];
+        const double hrr = node_r ? xn[i + 1] - xn[i] : 0.0;
+        const double mm = 0.5 * (hl + hrr);
+        const double cl = (1.0 / hl) / mm;
+        const double cr = node_r ? (1.0 / hrr) / mm : 0.0;
+        double t = cl * (dl / dv) * (f - fl) + cr * (dr / dv) * (f - fr);
+        t += a.gs_field_weight * fmax(w[u], 0.0) * f;
+        f = f + coef * t;
+      }
+    } else if (pc) {
+      double fl = __shfl_up(f, 1), fr = __shfl_down(f, 1);
+      if (act) {
+        const int i = u + 1;  // node of unknown u
+        const bool has_l = u > 0, has_r = u + 1 < n;
+        if (lane == 0 && has_l) fl = mix_residual(r[u - 1], pt[u - 1], a.transform, a.floor);
+        if (lane == 63 && has_r) fr = mix_residual(r[u + 1], pt[u + 1], a.transform, a.floor);
         fl = has_l ? fl : 0.0;
         fr = has_r ? fr : 0.0;
         const double hl = has_l ? xn[i] - xn[i - 1] : 0.0;
@@ -204,9 +233,7 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
         const double mm = 0.5 * (hl + hrr);
         const double cl = has_l ? (1.0 / hl) / mm : 0.0;
         const double cr = has_r ? (1.0 / hrr) / mm : 0.0;
-        double kh = (cl + cr) * hv - cl * fl - cr * fr;
-        if (wp) kh = (kh + a.warm_field_weight * fmax(w[u], 0.0) * hv) / dq;
-        f = f + coef * kh;
+        f = f + coef * ((cl + cr) * f - cl * fl - cr * fr);
       }
     }
     if (act) {

@@ -754,10 +754,10 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->backtrack = 0.0;
   o->max_halvings = 30;
   o->field_floor = 0.0;
-  o->warm_precond = 0.0;
-  o->warm_field_weight = 4.0;
-  o->warm_switch = 0;
-  o->warm_switch_growth = 0.0;
+  o->gs_precond = 0.0;
+  o->gs_field_weight = 1.0;
+  o->gs_switch = 0;
+  o->gs_switch_growth = 0.0;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -813,8 +813,8 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   if (o->history < 1 || o->history > MIX_MAX_HISTORY || o->max_iter < 1) return SCFT_E_ARG;
   if (!(o->precond >= 0.0) || !(o->backtrack >= 0.0) || o->max_halvings < 0) return SCFT_E_ARG;
   if (!(o->field_floor >= 0.0 && o->field_floor < 1.0)) return SCFT_E_ARG;
-  if (!(o->warm_precond >= 0.0) || !(o->warm_field_weight >= 0.0) || o->warm_switch < 0 ||
-      !(o->warm_switch_growth >= 0.0))
+  if (!(o->gs_precond >= 0.0) || !(o->gs_field_weight >= 0.0) || o->gs_switch < 0 ||
+      !(o->gs_switch_growth >= 0.0))
     return SCFT_E_ARG;
   SCFT_TRY(ensure_mixer(b, o->history + 1));
   const auto t0 = std::chrono::steady_clock::now();
@@ -850,10 +850,10 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.backtrack = o->backtrack;
   a.max_halvings = o->max_halvings;
   a.field_floor = o->field_floor;
-  a.warm_precond = o->warm_precond;
-  a.warm_field_weight = o->warm_field_weight;
-  a.warm_switch = o->warm_switch;
-  a.warm_switch_growth = o->warm_switch_growth;
+  a.gs_precond = o->gs_precond;
+  a.gs_field_weight = o->gs_field_weight;
+  a.gs_switch = o->gs_switch;
+  a.gs_switch_growth = o->gs_switch_growth;
   // fresh mixer state (a small kernel-free host copy of zeros is not needed:
   // the per-instance state array is rewritten from a pinned staging copy)
   if (static_cast<int>(b->ms0.size()) != b->n_inst)

@@ -47,7 +47,7 @@ constexpr unsigned ST_PIVOT = 4u;
 constexpr unsigned ST_MAXITER = 8u;
 constexpr unsigned ST_FALLBACK = 16u;
 constexpr unsigned ST_FAILED = 32u;
-constexpr unsigned ST_COLD_FORM = 64u;  // left the warm-start preconditioner
+constexpr unsigned ST_PLAIN_FORM = 64u;  // left the ground-state form
 // instances the march skips: converged, out of iterations, or given up by the
 // mixer (FAILED without a NaN: growth with no halvings left)
 constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER | ST_FAILED;
@@ -88,10 +88,10 @@ struct AndersonArgs {
   double backtrack;        // growth factor that triggers a halving (0: off)
   int max_halvings;
   double field_floor;      // w >= -field_floor * N after each update (0: off)
-  double warm_precond;     // warm-start preconditioner coefficient (0: off)
-  double warm_field_weight;  // its field screening weight c
-  int warm_switch;           // evaluations allowed in the warm form (0: no limit)
-  double warm_switch_growth; // leave it when ||r|| > growth * best (0: never)
+  double gs_precond;       // ground-state form coefficient (0: off)
+  double gs_field_weight;  // its field weight c
+  int gs_switch;           // evaluations allowed in that form (0: no limit)
+  double gs_switch_growth; // leave it when ||r|| > growth * best (0: never)
 };
 
 // Size bucket of an instance with n unknowns: the power of two >= n (at

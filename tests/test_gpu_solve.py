@@ -109,14 +109,18 @@ def test_field_floor_trajectory_matches_oracle():
             assert dw <= 1e-8
 
 
-@pytest.mark.parametrize("K,switch", [(2, 0), (6, 0), (6, 3)])
-def test_warm_preconditioner_trajectory_matches_oracle(K, switch):
-    """The warm-start form of the preconditioner (q-weighted Laplacian plus
-    the field's screening, anderson.hip) against orc_anderson's, over K
-    updates from starts near the solution -- the oracle's converged fields
-    with 1e-3 relative bumps, the regime the form is meant for -- on a stiff
-    graded instance and two uniform ones (bar 1e-8 relative); with
-    warm_switch = 3 both leave the warm form after the third evaluation."""
+@pytest.mark.parametrize("K,switch,start", [(2, 0, "near"), (6, 0, "near"), (6, 3, "near"),
+                                             (3, 0, "cold")])
+def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start):
+    """The ground-state form of the preconditioner (anderson.hip) against
+    orc_anderson's, over K updates, on a stiff graded instance and two
+    uniform ones (bar 1e-8 relative): from starts near the solution (the
+    oracle's converged fields with 1e-3 relative bumps, as a prediction) and
+    from w0; with gs_switch = 3 both leave the form for the plain one after
+    the third evaluation.  From w0 the comparison stops after 3 updates: on
+    the graded instance the fourth evaluation's residual is 2.9 and the
+    fifth's 1.3e8 (the oracle's own trajectory), an excursion through which
+    rounding differences of the two Gram systems grow to ~1e-5."""
     S = _S()
     c5 = WL.config5(n_inst=2048)
     c3 = WL.config3(n_inst=3, seed=7, Ns=(256,))
@@ -134,34 +138,34 @@ def test_warm_preconditioner_trajectory_matches_oracle(K, switch):
         assert sol["rc"] == 0
         x = xs[j]
         bump = np.cos(np.pi * x / x[-1]) + 0.5 * np.sin(3.0 * np.pi * x / x[-1])
-        w = sol["w"] * (1.0 + 1e-3 * bump)
+        w = sol["w"] * (1.0 + 1e-3 * bump) if start == "near" else w0
         w[0] = 0.0
         starts.append(w)
     o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0, max_iter=K + 1,
-                              precond=0.5, field_floor=0.5, warm_precond=0.5,
-                              warm_field_weight=4.0, warm_switch=switch)
+                              precond=0.5, field_floor=0.5, gs_precond=0.5,
+                              gs_field_weight=1.0, gs_switch=switch)
     o_cpu.max_iter = K
     b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
     out = b.solve([w[1:] for w in starts], o_dev)
     if switch:
-        assert ((out["status"] & S.ST_COLD_FORM) != 0).all()
+        assert ((out["status"] & S.ST_PLAIN_FORM) != 0).all()
     b.close()
     for j in range(3):
         ref = O.solve_anderson(xs[j], pts[j], Ns[j], starts[j], o_cpu, bc=O.BC_DN)
         plain = O.solve_anderson(xs[j], pts[j], Ns[j], starts[j],
-                                 O.anderson_opts(**{**_base_opts(K), "warm_precond": 0.0}),
+                                 O.anderson_opts(**{**_base_opts(K), "gs_precond": 0.0}),
                                  bc=O.BC_DN)
         dw = np.abs(out["w"][j] - ref["w"][1:]).max() / max(1.0, np.abs(ref["w"]).max())
         print(f"instance {j} M={len(xs[j])}: {K} updates, relative |w_dev - w_cpu| = {dw:.2e}",
               flush=True)
         assert dw <= 1e-8
-        assert not np.array_equal(ref["w"], plain["w"])  # the warm form was in effect
+        assert not np.array_equal(ref["w"], plain["w"])  # the form was in effect
 
 
 def _base_opts(K):
     return dict(history=16, beta=1.0, tol=1e-6, max_iter=K, step_cap=0.0, restart_factor=1e30,
                 regularization=1e-14, transform=2, floor=1e-8, precond=0.5, field_floor=0.5,
-                warm_field_weight=4.0)
+                gs_field_weight=1.0)
 
 
 def test_device_anderson_converges_config1():

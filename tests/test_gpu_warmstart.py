@@ -31,14 +31,15 @@ def test_warm_start_on_held_out_instances_as_specified():
     the batched solver converged (configuration-3 distribution), Adam at rate
     1e-3, batch 256, 50 epochs; then 2048 instances from disjoint seeds solved
     from w0 and from the prediction (device plan, outliers back to w0), with
-    the evaluation counts of both.  The prediction starts mix in the
-    warm-start form (history 8, plain form after 24 evaluations).  Both
-    solves converge every instance to the same solution.  Measured: mean 14.6
-    evaluations cold, 7.8 warm (training is deterministic for a seed)."""
+    the evaluation counts of both, each with the ground-state preconditioner
+    (the predicted starts with history 8 and the plain form after 24
+    evaluations).  Both solves converge every instance to the same
+    solution."""
     import scft_amd as S
     from scft_amd import warmstart as WS, workloads as WL
     kw = dict(beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0, transform=S.MIX_LOG,
-              precond=0.5, backtrack=1e30, check_every=4, floor=1e-8, field_floor=0.5)
+              precond=0.5, backtrack=1e30, check_every=4, floor=1e-8, field_floor=0.5,
+              gs_precond=0.5, gs_field_weight=1.0)
     opts = S.anderson_opts(history=16, **kw)
     X, Y = _train_fields(S, WS, WL, opts, 20000, 4242)
     ws = WS.WarmStart(S, seed=5)
@@ -57,8 +58,7 @@ def test_warm_start_on_held_out_instances_as_specified():
     prep = WS.Prepared(S, test)
     plan = ws.device_plan(b, prep)
     plan.predict()
-    wopts = S.anderson_opts(history=8, fallback_factor=10.0, warm_precond=0.5,
-                            warm_field_weight=4.0, warm_switch=24, **kw)
+    wopts = S.anderson_opts(history=8, fallback_factor=10.0, gs_switch=24, **kw)
     b.solve_resident(wopts)
     warm = b.download()
     plan.close()

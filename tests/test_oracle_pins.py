@@ -9,7 +9,11 @@
   count is round(ell ln(1 + L/ell) / h0) clamped to [63, 4095], and the nodes
   solve the recurrence x_{i+1} = x_i + h(x_i) written out independently.
 * The residual-form Anderson mixer (oracle_solvers.c:orc_anderson), which
-  the device mixer follows.  On an affine residual r(w) = b - A w with the
+  the device mixer follows, and its ground-state preconditioner (no
+  reference counterpart), pinned by the properties of -(1/d^2)(d^2 f')':
+  constants in its null space, self-adjoint and nonnegative in the product
+  weighted by the lumped masses and d^2, and Anderson's finite termination
+  with it as a fixed preconditioner.  On an affine residual r(w) = b - A w with the
   history at least n and no regularisation, Anderson mixing is GMRES in
   disguise (Walker & Ni, SIAM J. Numer. Anal. 49 (2011) 1715-1735, Thm 2.2):
   the iterate after n+1 evaluations is the exact fixed point, so the
@@ -169,47 +173,102 @@ def test_anderson_reference_linear_toy():
     np.testing.assert_allclose(w, [-4.0, 6.0], rtol=0, atol=1e-12)
 
 
-# ------------------------------------------------------------ warm-start form
-def _mixer_run(x, pt, N, w0, **kw):
-    o = O.anderson_opts(history=16, beta=1.0, tol=0.0, step_cap=0.0, regularization=0.0,
-                        transform=O.MIX_LOG, floor=1e-8, **kw)
-    return O.solve_anderson(x, pt, N, w0, o, bc=O.BC_DN)["w"]
+# ------------------------------------------------------------ ground-state form
+def _gs_operator(x, phi_t_unk, w0, a=0.5, c=0.0, floor=1e-8):
+    """P of the ground-state form, column by column: one plain-mixing step
+    (history 0, beta 1, plain transform) of orc_anderson on the constant
+    residual e_j returns w0 + P e_j."""
+    n = len(w0)
+    gs = np.zeros(3 * n)
+    O.lib().orc_gs_stencil(np.ascontiguousarray(x), len(x), O.BC_DN, gs)
+    P = np.empty((n, n))
+    for j in range(n):
+        e = np.zeros(n)
+        e[j] = 1.0
+
+        def F(nn, w, r, ctx, e=e):
+            for i in range(nn):
+                r[i] = e[i]
+            return 0
+
+        cb = O.RESIDUAL_FN(F)
+        opts = O.anderson_opts(history=0, beta=1.0, tol=0.0, max_iter=1, step_cap=0.0,
+                               regularization=0.0, transform=O.MIX_PLAIN, floor=floor,
+                               gs_precond=a, gs_field_weight=c)
+        opts.gstencil = gs.ctypes.data
+        w = np.array(w0, dtype=np.float64)
+        stt = O.AndersonStats(0, 0, 0.0, None, 0)
+        O.lib().orc_anderson(w, n, C.cast(cb, C.c_void_p), None, C.byref(opts),
+                             np.ascontiguousarray(phi_t_unk).ctypes.data, C.byref(stt))
+        P[:, j] = w - w0
+    return P
 
 
-def test_warm_preconditioner_reduces_to_the_plain_one():
-    """g = f + a d^-1 (K + c W+)(d f) with d constant (phi_t = 1 everywhere,
-    no wall layer) and c = 0 is the plain form g = f + a K f: the same
-    iterates bit for bit."""
-    x = O.mesh_uniform(41, 3.0)
-    pt = np.ones(41)
-    w0 = 0.3 * np.cos(2.0 * x)
-    w0[0] = 0.0
-    for k in (1, 2, 4):
-        plain = _mixer_run(x, pt, 128, w0, max_iter=k, precond=0.5)
-        warm = _mixer_run(x, pt, 128, w0, max_iter=k, precond=0.5, warm_precond=0.5,
-                          warm_field_weight=0.0)
-        assert np.array_equal(plain, warm)
-    # with the wall target the two differ (d varies next to the wall)
-    ptw = O.target_position(x, O.TAU, False)
-    a = _mixer_run(x, ptw, 128, w0, max_iter=2, precond=0.5)
-    b = _mixer_run(x, ptw, 128, w0, max_iter=2, precond=0.5, warm_precond=0.5,
-                   warm_field_weight=0.0)
-    assert not np.array_equal(a, b)
+def _graded_case(n=40):
+    x = WL.graded_mesh(3.0, 2e-3, 0.2)[: n + 1].copy()  # DN: unknowns at nodes 1..n
+    pt = O.target_position(x, O.TAU, False)[1:]
+    return x, pt
+
+
+def test_ground_state_form_maps_constants_to_constants():
+    """g = f + a [-(1/d^2)(d^2 f')'] with c = 0: a constant residual is mixed
+    unchanged, whatever d (here phi_t next to a wall, where d spans two
+    decades and the wall node's d is zero)."""
+    x, pt = _graded_case()
+    P = _gs_operator(x, pt, np.zeros(len(pt)))
+    np.testing.assert_allclose(P @ np.ones(len(pt)), np.ones(len(pt)), rtol=0, atol=1e-9)
+
+
+def test_ground_state_form_is_self_adjoint_in_the_weighted_product():
+    """-(1/d^2)(d^2 f')' is self-adjoint and nonnegative in the product
+    <f, g> = sum_u m_u d_u^2 f_u g_u (m_u the lumped mass (h_l + h_r)/2 of
+    node u + 1, the wall spacing included, h_r = 0 at the natural end):
+    W (P - I) is symmetric positive semidefinite.  A swapped ratio
+    d_u/d_{u-1}, a wrong mass or a sign error breaks it."""
+    x, pt = _graded_case()
+    n = len(pt)
+    P = _gs_operator(x, pt, np.zeros(n), a=0.5)
+    hl = np.diff(x)[:n]
+    hr = np.concatenate([np.diff(x)[1:n], [0.0]])
+    m = 0.5 * (hl + hr)
+    d = np.sqrt(np.maximum(pt, 1e-8))
+    A = np.diag(m * d * d) @ (P - np.eye(n))
+    scale = np.abs(A).max()
+    assert np.abs(A - A.T).max() <= 1e-12 * scale
+    ev = np.linalg.eigvalsh(0.5 * (A + A.T))
+    assert ev.min() >= -1e-12 * scale
+    # the coupling of neighbours u, u+1 is -a d_u d_{u+1} / h: nonzero, local
+    assert np.abs(np.triu(A, 2)).max() == 0.0
+    np.testing.assert_allclose(np.diag(A, 1), -0.5 * d[:-1] * d[1:] / np.diff(x)[1:n],
+                               rtol=1e-12)
+
+
+def test_ground_state_form_field_term():
+    """The field term adds a c max(w, 0) on the diagonal, w the current
+    iterate."""
+    x, pt = _graded_case(24)
+    rng = np.random.default_rng(8)
+    w0 = rng.uniform(-3.0, 5.0, len(pt))
+    P0 = _gs_operator(x, pt, w0, a=0.5, c=0.0)
+    P1 = _gs_operator(x, pt, w0, a=0.5, c=1.5)
+    # (differences of entries up to ~1e5: rounding at 1e-15 of that scale)
+    np.testing.assert_allclose(P1 - P0, np.diag(0.5 * 1.5 * np.maximum(w0, 0.0)), rtol=0,
+                               atol=1e-15 * np.abs(P0).max())
 
 
 @pytest.mark.parametrize("n", [6, 12])
-def test_warm_preconditioner_keeps_finite_termination(n):
-    """With c = 0 the warm form is a fixed linear operator P = I + a D^-1 K D,
-    so Anderson on the preconditioned affine residual P (b - A w) is still
-    GMRES and terminates by evaluation n+2 (plain transform here: the test
-    drives orc_anderson with an affine residual, phi_t only weights d)."""
+def test_ground_state_form_keeps_finite_termination(n):
+    """With c = 0 the form is a fixed linear operator P, so Anderson on the
+    preconditioned affine residual P (b - A w) is still GMRES and terminates
+    by evaluation n+2 (plain transform: the test drives orc_anderson with an
+    affine residual, phi_t only weights d)."""
     rng = np.random.default_rng(200 + n)
     Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
     A = Q @ np.diag(np.geomspace(0.2, 2.0, n)) @ Q.T
     b = rng.standard_normal(n)
     x = np.linspace(0.0, 1.0, n + 1)
-    st = np.zeros(3 * n)
-    O.lib().orc_precond_stencil(np.ascontiguousarray(x), n + 1, O.BC_DN, st)
+    gs = np.zeros(3 * n)
+    O.lib().orc_gs_stencil(np.ascontiguousarray(x), n + 1, O.BC_DN, gs)
     phi_t = np.geomspace(1e-4, 1.0, n)  # d spans two decades, as next to a wall
     calls = []
 
@@ -224,8 +283,8 @@ def test_warm_preconditioner_keeps_finite_termination(n):
     cb = O.RESIDUAL_FN(F)
     opts = O.anderson_opts(history=16, beta=1.0, tol=0.0, max_iter=n + 2, step_cap=0.0,
                            regularization=0.0, transform=O.MIX_PLAIN, precond=0.0,
-                           warm_precond=0.05, warm_field_weight=0.0)
-    opts.pstencil = st.ctypes.data
+                           gs_precond=0.05, gs_field_weight=0.0)
+    opts.gstencil = gs.ctypes.data
     w = np.zeros(n)
     stt = O.AndersonStats(0, 0, 0.0, None, 0)
     O.lib().orc_anderson(w, n, C.cast(cb, C.c_void_p), None, C.byref(opts),
