@@ -88,18 +88,19 @@ WORKLOADS = {
 
 
 # The ground-state form of the preconditioner, used by every leg:
-# g_u = f_u + 0.5 [ l (d_{u-1}/d_u)(f_u - f_{u-1}) + r (d_{u+1}/d_u)(f_u - f_{u+1})
-#                   + 1.0 max(w_u, 0) f_u ],  d = sqrt(phi_t),
-# the operator -(1/d^2)(d^2 f')' + w+.  On production-mix instances the
+# g_u = f_u + 0.45 [ l (d_{u-1}/d_u)(f_u - f_{u-1}) + r (d_{u+1}/d_u)(f_u - f_{u+1})
+#                    + 0.25 max(w_u, 0) f_u ],  d = sqrt(phi_t),
+# the operator -(1/d^2)(d^2 f')' + w+/4.  On production-mix instances the
 # preconditioned Jacobian has every eigenvalue within 0.14 of 1 with it.
-# CPU oracle, evaluations (mean / max): from the network's predictions 3.4 / 5
-# on 96 random instances and 4.0 / 6 on 48 stiff ones (dt/h0^2 >= 1.4e4),
-# against 8.4 / 15 for the earlier d^-1 (M_L^-1 K + 4 W+) d form; from w0
-# 13.7 / 21 against 20.1 / 27 for the plain form I + 0.6 M_L^-1 K (48
-# random) and 22.0 / 32 against 26.8 / 38 (48 stiff).  A field weight of 4
-# instead of 1 is as good from predictions and fails from w0.
-GS_PRECOND = 0.5
-GS_FIELD_WEIGHT = 1.0
+# Settings from whole-batch solves (production mix, seconds per solve from
+# w0 / from predictions): field weight 0, 0.25, 0.5, 0.75, 1 -> 1.17, 1.06,
+# 1.25, 1.46, 2.32 s from w0 (weight 1 leaves a tail to 110 evaluations),
+# 0.35-0.36 s from predictions at any weight; coefficient 0.4, 0.45, 0.5,
+# 0.55 (weight 0.25 / 0.5) -> 1.18, 1.05, 1.06, 2.67 s from w0, and 0.337
+# against 0.360 s from predictions at 0.45 and 0.5.  The 4096-instance mix
+# agrees (0.102-0.110 s from w0, 0.047-0.049 s from predictions).
+GS_PRECOND = 0.45
+GS_FIELD_WEIGHT = 0.25
 # Predicted starts: an instance not converged after 24 evaluations goes over
 # to the plain form with its current field (history cleared), a safety net
 # that no instance of either bench mix reaches.

@@ -191,10 +191,11 @@ typedef struct scft_anderson_opts {
                              l_u, r_u the linear-element Laplacian's
                              couplings to the neighbouring nodes: the
                              operator -(1/d^2)(d^2 f')' + c w+, which maps
-                             constants to themselves.  From predicted starts
-                             ~2.5x fewer evaluations than the plain form,
-                             from w0 ~1.5x fewer (0: off) */
-  double gs_field_weight; /* c of gs_precond (1) */
+                             constants to themselves.  With a = 0.45 and
+                             c = 0.25: from predicted starts ~2.4x fewer
+                             evaluations than the plain form, from w0 ~2x
+                             fewer (0: off) */
+  double gs_field_weight; /* c of gs_precond (0.25) */
   int gs_switch;          /* an instance leaves the ground-state form for
                              the plain one (history cleared,
                              SCFT_ST_PLAIN_FORM) after this many evaluations

@@ -755,7 +755,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->max_halvings = 30;
   o->field_floor = 0.0;
   o->gs_precond = 0.0;
-  o->gs_field_weight = 1.0;
+  o->gs_field_weight = 0.25;
   o->gs_switch = 0;
   o->gs_switch_growth = 0.0;
 }
