@@ -176,6 +176,11 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     an instance whose first residual exceeds 10x the batch median restarts
     from w0."""
     from scft_amd import warmstart as WS
+    if args.net_file and os.path.exists(args.net_file):
+        ws = WS.WarmStart(S, seed=11)
+        rec = ws.load(args.net_file)
+        log(f"bench: warm start: network loaded from {args.net_file} ({rec})")
+        return warm_solves(S, b, cfg, w0, args, ws, rec)
     t0 = time.perf_counter()
     Xs, Ys = [], []
     opts = solver_opts(S, args)
@@ -199,6 +204,17 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     t_train = time.perf_counter() - t0
     log(f"bench: warm start: {len(X)} training fields in {t_data:.1f} s, {len(hist)} epochs in "
         f"{t_train:.1f} s, loss {hist[0]:.3e} -> {hist[-1]:.3e}")
+    rec = {"training_fields": int(len(X)), "epochs": int(len(hist)),
+           "final_loss": round(float(hist[-1]), 6), "train_seconds": round(t_train, 2)}
+    if args.net_file:
+        ws.save(args.net_file, **rec)
+    return warm_solves(S, b, cfg, w0, args, ws, rec)
+
+
+def warm_solves(S, b, cfg, w0, args, ws, rec):
+    """Time K warm-started solves of the bench batch with the trained
+    network `ws` (its training record `rec`)."""
+    from scft_amd import warmstart as WS
     prep = WS.Prepared(S, cfg)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
@@ -232,7 +248,8 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
             f"{float(np.mean(res['iters'])):.1f}, max {int(np.max(res['iters']))}, {fb} fallbacks")
     # first residuals of the predicted and the cold fields (outside the timed steps)
     plan.predict()
-    first_warm = b.eval(b.download()["w"])["rnorm"]
+    w_pred = b.download()["w"]
+    first_warm = b.eval(w_pred)["rnorm"]
     first_cold = b.eval(w0)["rnorm"]
     plan.close()
     q = lambda a: [float(f"{v:.3g}") for v in np.quantile(a, [0.1, 0.5, 0.9])]
@@ -243,13 +260,14 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
         "cold_first_residual_q10_q50_q90": q(first_cold),
         "network": f"{WS.DIMS[0]} -> 4 x 256 tanh -> {WS.DIMS[-1]}, FP32, device "
                    "forward/backward/Adam kernels",
-        "training_fields": int(len(X)), "epochs": int(len(hist)), "batch": 256,
-        "lr_schedule": [list(st) for st in TRAIN_SCHEDULE],
-        "final_loss": round(float(hist[-1]), 6), "train_seconds": round(t_train, 2),
+        "training_fields": rec.get("training_fields"), "epochs": rec.get("epochs"),
+        "batch": 256, "lr_schedule": [list(st) for st in TRAIN_SCHEDULE],
+        "final_loss": rec.get("final_loss"), "train_seconds": rec.get("train_seconds"),
         "predict_ms_per_step": round(1e3 * tp / args.steps, 2),
         "fallbacks": int(np.sum((res["status"] & S.ST_FALLBACK) != 0)),
     }
-    return dict(res=res, total=total, value=res["converged"] * args.steps / total, info=info)
+    return dict(res=res, total=total, value=res["converged"] * args.steps / total, info=info,
+                starts=[np.concatenate([[0.0], v]) for v in w_pred])
 
 
 def solver_opts(S, args):
@@ -369,11 +387,13 @@ def stepper_compare(args):
             f"{n} instances, ||r|| of the reference field {out['rnorm'][0]:.2e}")
 
 
-def cpu_baseline(cfg, gpu_iters, args, threads, budget_s):
+def cpu_baseline(cfg, gpu_iters, args, threads, budget_s, starts=None, overrides=None):
     """The CPU oracle (same mixer, options and tolerance) solving a random
-    sample of the batch on `threads` OpenMP threads.  The sample is drawn
-    until its estimated CPU work reaches `budget_s`; the full-batch CPU time
-    is the sample's time scaled by the work ratio sum(iters * n * N)."""
+    sample of the batch on `threads` OpenMP threads, from `starts` (fields on
+    all nodes; default w0) with the solver settings plus `overrides`.  The
+    sample is drawn until its estimated CPU work reaches `budget_s`; the
+    full-batch CPU time is the sample's time scaled by the work ratio
+    sum(iters * n * N)."""
     import oracle as O
     O.build()
     n_inst = len(cfg["xs"])
@@ -392,11 +412,12 @@ def cpu_baseline(cfg, gpu_iters, args, threads, budget_s):
         idx.append(int(k))
         est += work[k] * c
     opts = O.anderson_opts(max_iter=args.max_iter, transform=O.MIX_LOG,
-                           **solver_settings(args.workload))
+                           **{**solver_settings(args.workload), **(overrides or {})})
+    starts = cfg["w0"] if starts is None else starts
     t0 = time.perf_counter()
     out = O.batch_solve([len(cfg["xs"][k]) for k in idx], [cfg["Ns"][k] for k in idx],
                         [cfg["xs"][k] for k in idx], [cfg["pts"][k] for k in idx],
-                        [cfg["w0"][k] for k in idx], opts, bc=O.BC_DN, threads=threads)
+                        [starts[k] for k in idx], opts, bc=O.BC_DN, threads=threads)
     dt = time.perf_counter() - t0
     sample_work = float(np.sum([out["iters"][i] * (len(cfg["xs"][k]) - 1) * cfg["Ns"][k]
                                 for i, k in enumerate(idx)]))
@@ -471,6 +492,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-warm", action="store_true", help="cold start only")
+    ap.add_argument("--net-file", default=None,
+                    help="warm-start network file: loaded if it exists (no training), "
+                         "else written after training")
+    ap.add_argument("--no-cold", action="store_true",
+                    help="skip the cold leg (profiling runs; no cold figures in the line)")
     ap.add_argument("--train-inst", type=int, default=40000,
                     help="converged fields the warm-start network is trained on")
     ap.add_argument("--roofline-only", action="store_true",
@@ -549,13 +575,15 @@ def main():
 
     # ---- cold start: every instance from w0 (the reference's initial guess)
     opts = solver_opts(S, args)
-    for i in range(args.warmup):
+    if args.no_cold and args.no_warm:
+        sys.exit("bench.py: --no-cold with --no-warm leaves nothing to time")
+    for i in range(0 if args.no_cold else args.warmup):
         b.upload(w0)
         r = b.solve_resident(opts)
         log(f"bench: cold warmup {i}: {r['seconds']:.2f} s, {r['sweeps']} sweeps")
     total_cold = 0.0
     res_cold = None
-    for i in range(args.steps):
+    for i in range(0 if args.no_cold else args.steps):
         b.upload(w0)
         r = b.solve_resident(opts)
         total_cold += r["seconds"]
@@ -563,7 +591,7 @@ def main():
         log(f"bench: cold step {i}: {r['seconds']:.3f} s, {r['sweeps']} sweeps, "
             f"{res_cold['converged']}/{args.n_inst} converged, mean iters "
             f"{float(np.mean(res_cold['iters'])):.1f}, max {int(np.max(res_cold['iters']))}")
-    value_cold = res_cold["converged"] * args.steps / total_cold
+    value_cold = res_cold["converged"] * args.steps / total_cold if res_cold else None
 
     # ---- warm start: the MLP trained on fields the batched solver produced
     warm = None
@@ -596,7 +624,7 @@ def main():
             "mean_iters": round(float(np.mean(res["iters"])), 2),
             "max_iters": int(np.max(res["iters"])),
             "solve_gbs_algorithmic": round(gb_eff, 1),
-            "cold_start": {
+            "cold_start": None if res_cold is None else {
                 "value": round(value_cold, 3), "unit": "solves/s",
                 "ms_per_step": round(1e3 * total_cold / args.steps, 3),
                 "converged": int(res_cold["converged"]),
@@ -616,29 +644,41 @@ def main():
     }
     if warm:
         out["config"]["warm_start"] = warm["info"]
-        out["config"]["warm_over_cold"] = round(value / value_cold, 3)
+        out["config"]["warm_over_cold"] = round(value / value_cold, 3) if value_cold else None
     out["config"]["audit"] = audit(S, cfg, b, res)
     log(f"bench: audit {out['config']['audit']}")
-    if not args.no_cpu:
+    if not args.no_cpu and res_cold is not None:
         # the host cores the GPU box allots to one GPU (OMP_NUM_THREADS there;
         # the whole host may show more): every one of them
         affinity = len(os.sched_getaffinity(0))
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or affinity
         threads = max(1, min(threads, affinity))
-        cb = cpu_baseline(cfg, res_cold["iters"], args, threads, args.cpu_budget)
+        def sample_text(c, what):
+            return (f"{c['sample']} random instances of the same batch solved by the CPU "
+                    f"oracle from {what} (same mixer, options, tolerance) in "
+                    f"{c['seconds']:.1f} s on {threads} threads, {c['converged']} converged; "
+                    f"full batch {c['full_time']:.0f} s by work ratio sum(iters*n*N)")
+        cbc = cpu_baseline(cfg, res_cold["iters"], args, threads,
+                           args.cpu_budget / 2 if warm else args.cpu_budget)
+        if warm:  # the measured workload: the same predicted fields, the warm leg's options
+            cb = cpu_baseline(cfg, res["iters"], args, threads, args.cpu_budget,
+                              starts=warm["starts"],
+                              overrides=dict(history=WARM_HISTORY, gs_switch=GS_SWITCH))
+            what = "the network's predicted fields (no fallback)"
+        else:
+            cb, what = cbc, "w0"
         c1 = config1_single_thread(S, WL)
         out["cpu_baseline"] = {
             "value": round(cb["value"], 4), "unit": "solves/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{cb['sample']} random instances of the same batch solved by the CPU "
-                      f"oracle (same mixer, options, tolerance) in {cb['seconds']:.1f} s on "
-                      f"{threads} threads, {cb['converged']} converged; full batch "
-                      f"{cb['full_time']:.0f} s by work ratio sum(iters*n*N)",
+            "kind": "port", "sample": sample_text(cb, what),
             "host_cpu_count": os.cpu_count(), "host_affinity": affinity,
             "gpu_over_cpu": round(value / cb["value"], 1),
+            "cold": {"value": round(cbc["value"], 4), "unit": "solves/s",
+                     "sample": sample_text(cbc, "w0"),
+                     "gpu_over_cpu": round(value_cold / cbc["value"], 1)},
             "config1_single_thread": c1,
         }
-        log(f"bench: cpu baseline {cb}; config 1 {c1}")
+        log(f"bench: cpu baseline {cb}; cold {cbc}; config 1 {c1}")
     b.close()
     print(json.dumps(out), flush=True)
 

@@ -214,6 +214,22 @@ class WarmStart:
             counts[i] = k
         return self.S.WarmPlan(batch, self.mlp, prep.F, prep.grids, NODE_IDX, W_SCALE, counts)
 
+    def save(self, path: str, **meta):
+        """Parameters, normalisation and shape in an .npz (no pickled
+        objects); `meta` adds scalar entries (training record)."""
+        xm, xs, ym, ys = self.mlp._norm
+        np.savez(path, dims=np.asarray(DIMS), params=self.mlp.params(), x_mean=xm, x_std=xs,
+                 y_mean=ym, y_std=ys, **{k: np.asarray(v) for k, v in meta.items()})
+
+    def load(self, path: str) -> dict:
+        """The network saved by `save`; returns its scalar entries."""
+        z = np.load(path, allow_pickle=False)
+        if list(z["dims"]) != list(DIMS):
+            raise ValueError(f"{path}: network shape {list(z['dims'])}, expected {DIMS}")
+        self.mlp.set_params(z["params"])
+        self.mlp.set_normalisation(z["x_mean"], z["x_std"], z["y_mean"], z["y_std"])
+        return {k: z[k].item() for k in z.files if z[k].ndim == 0}
+
     def close(self):
         self.mlp.close()
 
