@@ -10,8 +10,11 @@
 // accumulation on the matrix cores (v_mfma_f32_16x16x4f32), and a fused
 // epilogue (bias + tanh for hidden layers, bias for the output layer, the
 // tanh derivative for back-propagated deltas, or nothing).  Workgroups of 4
-// waves own a 64 x 64 tile of C; each wave a 32 x 32 quarter as 2 x 2 blocks
-// of 16 x 16; K advances in chunks of 16 staged in LDS.
+// waves own a 32 x 32 tile of C, each wave one 16 x 16 block; K advances in
+// chunks of 64 staged in LDS, each operand loaded along whichever of its
+// strides is 1.  The training products are 256 x 256 x 256 at batch 256:
+// small tiles give 64 workgroups for them, and deep chunks four round trips
+// to memory per tile instead of sixteen.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,8 +31,8 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-constexpr int TILE = 64;
-constexpr int KC = 16;  // k chunk staged per iteration
+constexpr int TILE = 32;
+constexpr int KC = 64;  // k chunk staged per iteration
 enum Epilogue { EP_NONE = 0, EP_BIAS = 1, EP_BIAS_TANH = 2, EP_DTANH = 3 };
 
 struct GemmArgs {
@@ -51,24 +54,24 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int m0 = blockIdx.y * TILE, n0 = blockIdx.x * TILE;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave's quarter
-  floatx4 acc[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int wm = (wave >> 1) * 16, wn = (wave & 1) * 16;  // the wave's block
+  // consecutive threads walk the operand's contiguous index
+  const bool a_kfast = g.a_k == 1, b_nfast = g.b_n == 1;
+  floatx4 acc = floatx4{0.f, 0.f, 0.f, 0.f};
 
   for (int k0 = 0; k0 < g.K; k0 += KC) {
-    // stage A[m0:m0+64, k0:k0+16] and B[k0:k0+16, n0:n0+64], zero outside
+    // stage A[m0:m0+32, k0:k0+64] and B[k0:k0+64, n0:n0+32], zero outside
 #pragma unroll
     for (int e = tid; e < TILE * KC; e += 256) {
-      const int mm = e / KC, kk = e % KC;
+      const int mm = a_kfast ? e / KC : e % TILE;
+      const int kk = a_kfast ? e % KC : e / TILE;
       const int m = m0 + mm, k = k0 + kk;
       As[mm][kk] = (m < g.M && k < g.K) ? g.A[m * g.a_m + k * g.a_k] : 0.f;
     }
 #pragma unroll
     for (int e = tid; e < TILE * KC; e += 256) {
-      const int kk = e / TILE, nn = e % TILE;
+      const int kk = b_nfast ? e / TILE : e % KC;
+      const int nn = b_nfast ? e % TILE : e / KC;
       const int k = k0 + kk, n = n0 + nn;
       Bs[kk][nn] = (n < g.N && k < g.K) ? g.B[k * g.b_k + n * g.b_n] : 0.f;
     }
@@ -78,50 +81,43 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     // 4 (l / 16) + 0..3 of column l % 16.
     const int r = lane & 15, q = lane >> 4;
 #pragma unroll
-    for (int ks = 0; ks < KC; ks += 4) {
-      float a[2], b[2];
-#pragma unroll
-      for (int i = 0; i < 2; i++) a[i] = As[wm + 16 * i + r][ks + q];
-#pragma unroll
-      for (int j = 0; j < 2; j++) b[j] = Bs[ks + q][wn + 16 * j + r];
-#pragma unroll
-      for (int i = 0; i < 2; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
-    }
+    for (int ks = 0; ks < KC; ks += 4)
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(As[wm + r][ks + q], Bs[ks + q][wn + r], acc,
+                                                 0, 0, 0);
     __syncthreads();
   }
 
   const int col = lane & 15, rq = (lane >> 4) * 4;
 #pragma unroll
-  for (int i = 0; i < 2; i++)
-#pragma unroll
-    for (int j = 0; j < 2; j++)
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int m = m0 + wm + 16 * i + rq + e;
-        const int n = n0 + wn + 16 * j + col;
-        if (m >= g.M || n >= g.N) continue;
-        float v = acc[i][j][e];
-        if (g.epilogue == EP_BIAS || g.epilogue == EP_BIAS_TANH) v += g.bias[n];
-        if (g.epilogue == EP_BIAS_TANH) v = tanhf(v);
-        if (g.epilogue == EP_DTANH) {
-          const float a = g.act[m * g.ldc + n];
-          v *= 1.f - a * a;
-        }
-        g.C[m * g.ldc + n] = v;
-      }
+  for (int e = 0; e < 4; e++) {
+    const int m = m0 + wm + rq + e;
+    const int n = n0 + wn + col;
+    if (m >= g.M || n >= g.N) continue;
+    float v = acc[e];
+    if (g.epilogue == EP_BIAS || g.epilogue == EP_BIAS_TANH) v += g.bias[n];
+    if (g.epilogue == EP_BIAS_TANH) v = tanhf(v);
+    if (g.epilogue == EP_DTANH) {
+      const float a = g.act[m * g.ldc + n];
+      v *= 1.f - a * a;
+    }
+    g.C[m * g.ldc + n] = v;
+  }
 }
 
-// gb[o] = sum_b delta[b][o]  (one thread per output, batch loop)
-__global__ void colsum_kernel(const float* __restrict__ delta, int batch, int dout,
-                              float* __restrict__ gb) {
-  const int o = blockIdx.x * blockDim.x + threadIdx.x;
-  if (o >= dout) return;
+// gb[o] = sum_b delta[b][o]: a workgroup takes 64 columns, its 4 waves
+// every 4th row each (lanes on consecutive columns, loads coalesced), then
+// the 4 partial sums meet in LDS
+__global__ void __launch_bounds__(256) colsum_kernel(const float* __restrict__ delta, int batch,
+                                                     int dout, float* __restrict__ gb) {
+  __shared__ float part[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int o = blockIdx.x * 64 + lane;
   float s = 0.f;
-  for (int b = 0; b < batch; b++) s += delta[(long long)b * dout + o];
-  gb[o] = s;
+  if (o < dout)
+    for (int b = wave; b < batch; b += 4) s += delta[(long long)b * dout + o];
+  part[wave][lane] = s;
+  __syncthreads();
+  if (wave == 0 && o < dout) gb[o] = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
 }
 
 // delta = 2 (y - t) / (batch * dout); loss partial sums into *loss (FP64)
@@ -272,7 +268,7 @@ int backward(scft_mlp* m, int batch, bool clear_loss = true) {
     // gW[o][i] = sum_b delta[b][o] act_l[b][i]
     MLPCHK(gemm(m->stream, delta, 1, dout, m->d_act[l], din, 1, gW, din, dout, din, batch,
                 EP_NONE, nullptr, nullptr));
-    hipLaunchKernelGGL(colsum_kernel, dim3((dout + 255) / 256), dim3(256), 0, m->stream, delta,
+    hipLaunchKernelGGL(colsum_kernel, dim3((dout + 63) / 64), dim3(256), 0, m->stream, delta,
                        batch, dout, gb);
     MLPCHK(hipGetLastError());
     if (l > 0) {  // delta_prev[b][i] = sum_o delta[b][o] W[o][i] * (1 - act_l[b][i]^2)
