@@ -110,3 +110,32 @@ def test_device_warm_plan_matches_host_mapping():
     print(f"device vs host prediction: max relative difference {err.max():.2e} over "
           f"{len(host)} unknowns", flush=True)
     assert err.max() <= 2e-6
+
+
+def test_saved_network_predicts_the_same(tmp_path):
+    """WarmStart.save / load: parameters, normalisation and shape in an .npz
+    (no pickled objects); the reloaded network predicts bit for bit the
+    same fields, and a file of another shape is refused."""
+    import scft_amd as S
+    from scft_amd import warmstart as WS, workloads as WL
+    cfg = WL.config5(n_inst=64, seed=41)
+    prep = WS.Prepared(S, cfg)
+    rng = np.random.default_rng(9)
+    X = rng.standard_normal((512, WS.DIMS[0])).astype(np.float32)
+    Y = rng.standard_normal((512, WS.DIMS[-1])).astype(np.float32)
+    a = WS.WarmStart(S, seed=3)
+    a.fit(X, Y, epochs=2, batch=128, lr=1e-3, seed=4)
+    path = str(tmp_path / "net.npz")
+    a.save(path, training_fields=512, epochs=2)
+    b = WS.WarmStart(S, seed=99)  # different initial weights
+    rec = b.load(path)
+    assert rec == {"training_fields": 512, "epochs": 2}
+    assert np.array_equal(a.predict_unknowns(prep), b.predict_unknowns(prep))
+    z = dict(np.load(path))
+    z["dims"] = np.asarray([WS.DIMS[0], 128, WS.DIMS[-1]])
+    bad = str(tmp_path / "bad.npz")
+    np.savez(bad, **z)
+    with pytest.raises(ValueError):
+        b.load(bad)
+    a.close()
+    b.close()
