@@ -162,6 +162,51 @@ def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start):
         assert not np.array_equal(ref["w"], plain["w"])  # the form was in effect
 
 
+def test_gs_preconditioner_trajectory_matches_oracle_dd():
+    """The ground-state form with both walls (DD, the reference's own
+    boundary conditions): the last unknown's right neighbour is the wall
+    node, whose term vanishes as the first unknown's left one does.  A
+    uniform and a graded mesh (graded toward both walls by mirroring), 4
+    updates from w0 and from a bumped converged field, bar 1e-8 relative."""
+    S = _S()
+    xu = O.mesh_uniform(301, 3.72)
+    half = WL.graded_mesh(1.86, 2e-3, 0.2)
+    xg = np.concatenate([half, 3.72 - half[-2::-1]])
+    xs = [xu, xg]
+    Ns = [512, 1024]
+    pts = [O.target_position(x, O.TAU, True) for x in xs]
+    K = 4
+    for start in ("cold", "near"):
+        starts = []
+        for j, x in enumerate(xs):
+            w0 = WL.w0_field(x, 50 + j)
+            w0 = w0 + w0[::-1]  # both walls
+            w0[0] = w0[-1] = 0.0
+            if start == "near":
+                sol = O.solve_anderson(x, pts[j], Ns[j], w0,
+                                       O.anderson_opts(**{**_base_opts(400), "tol": 1e-9,
+                                                          "gs_precond": 0.45,
+                                                          "gs_field_weight": 0.25}),
+                                       bc=O.BC_DD)
+                assert sol["rc"] == 0
+                w0 = sol["w"] * (1.0 + 1e-3 * np.cos(np.pi * x / x[-1]))
+                w0[0] = w0[-1] = 0.0
+            starts.append(w0)
+        o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0,
+                                  max_iter=K + 1, precond=0.5, field_floor=0.5,
+                                  gs_precond=0.45, gs_field_weight=0.25)
+        o_cpu.max_iter = K
+        b = S.Batch(xs, Ns, [p[1:-1] for p in pts], bc=S.BC_DD)
+        out = b.solve([w[1:-1] for w in starts], o_dev)
+        b.close()
+        for j in range(2):
+            ref = O.solve_anderson(xs[j], pts[j], Ns[j], starts[j], o_cpu, bc=O.BC_DD)
+            dw = np.abs(out["w"][j] - ref["w"][1:-1]).max() / max(1.0, np.abs(ref["w"]).max())
+            print(f"DD {start} instance {j} M={len(xs[j])}: {K} updates, relative "
+                  f"|w_dev - w_cpu| = {dw:.2e}", flush=True)
+            assert dw <= 1e-8
+
+
 def _base_opts(K):
     return dict(history=16, beta=1.0, tol=1e-6, max_iter=K, step_cap=0.0, restart_factor=1e30,
                 regularization=1e-14, transform=2, floor=1e-8, precond=0.5, field_floor=0.5,
