@@ -202,6 +202,14 @@ typedef struct scft_anderson_opts {
                              without converging (0: never) ... */
   double gs_switch_growth;/* ... or when its ||r||_inf exceeds this factor
                              times its best so far (0: never) */
+  double gs_shift;        /* g > 0: the rational ground-state form
+                               g = (1 - a) f + a G f + a g (g I + G)^{-1} f,
+                             G the operator above: its symbol follows the
+                             inverse Debye function of the chain's response
+                             to ~2% (a = 1/2, g = 3) where the linear form
+                             f + a G f follows it to ~13%.  ~20% fewer
+                             evaluations from predicted starts, none fewer
+                             from w0 (0: the linear form) */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

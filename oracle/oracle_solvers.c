@@ -256,7 +256,36 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
      *                   + r_u (d_{u+1}/d_u)(f_u - f_{u+1}) + c max(w_u,0) f_u ]
      * with d = sqrt(max(phi_t, floor)) and d = 0 beyond the unknowns (the
      * wall nodes; a natural end has r_u = 0) */
-    if (gs_on) {
+    if (gs_on && o->gs_shift > 0.0) {
+      /* the rational form (1 - a) f + a G f + a g (g I + G)^{-1} f: G is
+       * tridiagonal with |lower| + |upper| = its diagonal minus c w+, so
+       * g I + G is strictly diagonally dominant and Thomas needs no pivots */
+      const double *gs = o->gstencil;
+      const double a = o->gs_precond, g = o->gs_shift;
+      double *lo = (double *)malloc(sizeof(double) * 4 * (size_t)n);
+      double *up = lo + n, *cp = up + n, *y = cp + n;
+      for (int i = 0; i < n; i++) {
+        const double di = sqrt(fmax(phi_t_unk[i], o->floor));
+        const double dl = i > 0 ? sqrt(fmax(phi_t_unk[i - 1], o->floor)) : 0.0;
+        const double dr = (i + 1 < n) ? sqrt(fmax(phi_t_unk[i + 1], o->floor)) : 0.0;
+        lo[i] = gs[3 * i] * (dl / di);
+        up[i] = gs[3 * i + 2] * (dr / di);
+        const double fl = i > 0 ? r[i - 1] : 0.0, fr = (i + 1 < n) ? r[i + 1] : 0.0;
+        double t = lo[i] * (r[i] - fl) + up[i] * (r[i] - fr);
+        t += o->gs_field_weight * fmax(w[i], 0.0) * r[i];
+        wn[i] = (1.0 - a) * r[i] + a * t;
+      }
+      for (int i = 0; i < n; i++) { /* forward sweep of (g I + G) y = f */
+        const double diag = g + lo[i] + up[i] + o->gs_field_weight * fmax(w[i], 0.0);
+        const double den = i > 0 ? diag - (-lo[i]) * cp[i - 1] : diag;
+        cp[i] = -up[i] / den;
+        y[i] = (i > 0 ? r[i] - (-lo[i]) * y[i - 1] : r[i]) / den;
+      }
+      for (int i = n - 2; i >= 0; i--) y[i] -= cp[i] * y[i + 1];
+      for (int i = 0; i < n; i++) wn[i] += a * g * y[i];
+      free(lo);
+      memcpy(r, wn, sizeof(double) * n);
+    } else if (gs_on) {
       const double *gs = o->gstencil;
       for (int i = 0; i < n; i++) {
         const double di = sqrt(fmax(phi_t_unk[i], o->floor));

@@ -46,6 +46,14 @@ namespace scft {
 constexpr int AM_THREADS = 256;
 constexpr int AM_MAXH = 16;  // previous iterates supported (history <= 16)
 
+// lane l's value of a double, l wave-uniform
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane(static_cast<unsigned>(b), l);
+  const unsigned hi = __builtin_amdgcn_readlane(static_cast<unsigned>(b >> 32), l);
+  return __builtin_bit_cast(double, (static_cast<unsigned long long>(hi) << 32) | lo);
+}
+
 __device__ __forceinline__ double mix_residual(double r, double pt, int transform,
                                                double floor) {
   if (transform == 0) return r;
@@ -183,9 +191,20 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // with d = sqrt(max(phi_t, floor)), zero beyond the unknowns (the wall
   // nodes), l_u = (1/h_l)/m_u, r_u = (1/h_r)/m_u, m_u = (h_l + h_r)/2 over
   // the mesh's nodes (a natural end has no right neighbour): the operator
-  // -(1/d^2)(d^2 f')' + c w+, which maps constants to themselves
+  // -(1/d^2)(d^2 f')' + c w+, which maps constants to themselves.  With
+  // gs_shift = g > 0 the rational form
+  //   g = (1 - a_g) f + a_g G f + a_g g (g I + G)^-1 f
+  // whose symbol (1 - a) + a x + a g / (g + x) follows 1/g_D(x), the
+  // inverse Debye function of the chain's linear response, to ~2% for
+  // a = 1/2, g = 3 (the linear form: to ~13%); (g I + G) y = f is solved
+  // below by the Thomas recurrence, G's rows being kept for it in `tri`.
   const double coef = gs ? a.gs_precond : a.precond;
   const bool pc = coef > 0.0 && (gs || n > 1);
+  const bool rational = pc && gs && a.gs_shift > 0.0;
+  double* __restrict__ t_lo = rational ? a.tri + d.uo : nullptr;
+  double* __restrict__ t_dg = rational ? t_lo + a.tri_stride : nullptr;
+  double* __restrict__ t_up = rational ? t_dg + a.tri_stride : nullptr;
+  double* __restrict__ t_rh = rational ? t_up + a.tri_stride : nullptr;
   // (whole-wave trips: the neighbours' values come from the lanes beside,
   // and only a wave's edge lanes evaluate theirs again)
   for (int u0 = 0; u0 < n; u0 += AM_THREADS) {
@@ -215,9 +234,19 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
         const double mm = 0.5 * (hl + hrr);
         const double cl = (1.0 / hl) / mm;
         const double cr = node_r ? (1.0 / hrr) / mm : 0.0;
-        double t = cl * (dl / dv) * (f - fl) + cr * (dr / dv) * (f - fr);
-        t += a.gs_field_weight * fmax(w[u], 0.0) * f;
-        f = f + coef * t;
+        const double lo = cl * (dl / dv), up = cr * (dr / dv);
+        const double fw = a.gs_field_weight * fmax(w[u], 0.0);
+        double t = lo * (f - fl) + up * (f - fr);
+        t += fw * f;
+        if (rational) {
+          t_lo[u] = lo;
+          t_dg[u] = a.gs_shift + lo + up + fw;
+          t_up[u] = up;
+          t_rh[u] = f;
+          f = (1.0 - coef) * f + coef * t;
+        } else {
+          f = f + coef * t;
+        }
       }
     } else if (pc) {
       double fl = __shfl_up(f, 1), fr = __shfl_down(f, 1);
@@ -242,6 +271,62 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
     }
   }
   __syncthreads();
+
+  if (rational) {
+    // (g I + G) y = f by Thomas: g I + G is strictly diagonally dominant, so
+    // no pivoting.  Wave 0 walks the rows in blocks of 64: each lane loads
+    // one row, every lane follows the (wave-uniform) recurrence through
+    // readlane, and the lane of a row keeps its result.  The forward sweep
+    // leaves cp in t_up and the modified right side in t_rh, the backward
+    // sweep y in t_rh.
+    if (tid < 64) {
+      const int lane = tid;
+      double cpv = 0.0, yv = 0.0;
+      for (int b0 = 0; b0 < n; b0 += 64) {
+        const int u = b0 + lane;
+        const bool ok = u < n;
+        const double Lo = ok ? t_lo[u] : 0.0, Dg = ok ? t_dg[u] : 1.0;
+        const double Up = ok ? t_up[u] : 0.0, Rh = ok ? t_rh[u] : 0.0;
+        const int cnt = min(64, n - b0);
+        double mycp = 0.0, myy = 0.0;
+        for (int j = 0; j < cnt; j++) {
+          const double lo = readlane_f64(Lo, j), dg = readlane_f64(Dg, j);
+          const double up = readlane_f64(Up, j), rh = readlane_f64(Rh, j);
+          const bool first = b0 + j == 0;
+          const double den = first ? dg : dg - (-lo) * cpv;
+          cpv = -up / den;
+          yv = (first ? rh : rh - (-lo) * yv) / den;
+          if (lane == j) {
+            mycp = cpv;
+            myy = yv;
+          }
+        }
+        if (ok) {
+          t_up[u] = mycp;
+          t_rh[u] = myy;
+        }
+      }
+      double ynext = 0.0;
+      for (int b0 = ((n - 1) / 64) * 64; b0 >= 0; b0 -= 64) {
+        const int u = b0 + lane;
+        const bool ok = u < n;
+        const double Cp = ok ? t_up[u] : 0.0, Yv = ok ? t_rh[u] : 0.0;
+        const int cnt = min(64, n - b0);
+        double myy = Yv;
+        for (int j = cnt - 1; j >= 0; j--) {
+          double yi = readlane_f64(Yv, j);
+          if (b0 + j < n - 1) yi = yi - readlane_f64(Cp, j) * ynext;
+          ynext = yi;
+          if (lane == j) myy = yi;
+        }
+        if (ok) t_rh[u] = myy;
+      }
+    }
+    __syncthreads();
+    for (int u = tid; u < n; u += AM_THREADS)
+      hr[(long long)cur * n + u] += coef * a.gs_shift * t_rh[u];
+    __syncthreads();
+  }
 
   // ----------------------------------------------------- Gram system
   // U_ij = sum_u dv_i dv_j and b_i = sum_u dv_i r_k with dv_i = r_k - r_{k-1-i}

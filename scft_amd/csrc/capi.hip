@@ -108,6 +108,7 @@ struct scft_batch {
   // mixer
   int hist_S = 0;
   double *d_hw = nullptr, *d_hr = nullptr, *d_wcold = nullptr;
+  double* d_tri = nullptr;  // rational preconditioner rows, 4 x total_u (on demand)
   MixState* d_ms = nullptr;
   int* d_iters = nullptr;
   int* d_mix_order = nullptr;  // instances by decreasing n (mixer dispatch order)
@@ -445,6 +446,7 @@ extern "C" int scft_batch_destroy(scft_batch* b) {
   dfree(b->d_hw);
   dfree(b->d_hr);
   dfree(b->d_wcold);
+  dfree(b->d_tri);
   dfree(b->d_ms);
   dfree(b->d_iters);
   dfree(b->d_mix_order);
@@ -758,6 +760,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->gs_field_weight = 0.25;
   o->gs_switch = 0;
   o->gs_switch_growth = 0.0;
+  o->gs_shift = 0.0;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -814,9 +817,13 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   if (!(o->precond >= 0.0) || !(o->backtrack >= 0.0) || o->max_halvings < 0) return SCFT_E_ARG;
   if (!(o->field_floor >= 0.0 && o->field_floor < 1.0)) return SCFT_E_ARG;
   if (!(o->gs_precond >= 0.0) || !(o->gs_field_weight >= 0.0) || o->gs_switch < 0 ||
-      !(o->gs_switch_growth >= 0.0))
+      !(o->gs_switch_growth >= 0.0) || !(o->gs_shift >= 0.0))
     return SCFT_E_ARG;
   SCFT_TRY(ensure_mixer(b, o->history + 1));
+  if (o->gs_shift > 0.0 && o->gs_precond > 0.0 && !b->d_tri) {
+    drop_sweep_graph(b);
+    SCFT_TRY(dalloc(&b->d_tri, 4 * b->total_u));
+  }
   const auto t0 = std::chrono::steady_clock::now();
   const int nb = static_cast<int>(b->buckets.size());
   AndersonArgs a;
@@ -854,6 +861,9 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.gs_field_weight = o->gs_field_weight;
   a.gs_switch = o->gs_switch;
   a.gs_switch_growth = o->gs_switch_growth;
+  a.gs_shift = o->gs_precond > 0.0 ? o->gs_shift : 0.0;
+  a.tri = b->d_tri;
+  a.tri_stride = b->total_u;
   // fresh mixer state (a small kernel-free host copy of zeros is not needed:
   // the per-instance state array is rewritten from a pinned staging copy)
   if (static_cast<int>(b->ms0.size()) != b->n_inst)

@@ -92,6 +92,9 @@ struct AndersonArgs {
   double gs_field_weight;  // its field weight c
   int gs_switch;           // evaluations allowed in that form (0: no limit)
   double gs_switch_growth; // leave it when ||r|| > growth * best (0: never)
+  double gs_shift;         // g > 0: the rational ground-state form (0: linear)
+  double* tri;             // 4 x tri_stride doubles: the rational form's rows
+  long long tri_stride;    // (lower, diagonal, upper, right side), at desc.uo
 };
 
 // Size bucket of an instance with n unknowns: the power of two >= n (at

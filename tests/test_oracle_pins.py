@@ -174,7 +174,7 @@ def test_anderson_reference_linear_toy():
 
 
 # ------------------------------------------------------------ ground-state form
-def _gs_operator(x, phi_t_unk, w0, a=0.5, c=0.0, floor=1e-8):
+def _gs_operator(x, phi_t_unk, w0, a=0.5, c=0.0, floor=1e-8, shift=0.0):
     """P of the ground-state form, column by column: one plain-mixing step
     (history 0, beta 1, plain transform) of orc_anderson on the constant
     residual e_j returns w0 + P e_j."""
@@ -194,7 +194,7 @@ def _gs_operator(x, phi_t_unk, w0, a=0.5, c=0.0, floor=1e-8):
         cb = O.RESIDUAL_FN(F)
         opts = O.anderson_opts(history=0, beta=1.0, tol=0.0, max_iter=1, step_cap=0.0,
                                regularization=0.0, transform=O.MIX_PLAIN, floor=floor,
-                               gs_precond=a, gs_field_weight=c)
+                               gs_precond=a, gs_field_weight=c, gs_shift=shift)
         opts.gstencil = gs.ctypes.data
         w = np.array(w0, dtype=np.float64)
         stt = O.AndersonStats(0, 0, 0.0, None, 0)
@@ -291,3 +291,44 @@ def test_ground_state_form_keeps_finite_termination(n):
                          np.ascontiguousarray(phi_t).ctypes.data, C.byref(stt))
     assert calls[n + 1] <= 1e-8 * np.abs(b).max(), calls
     np.testing.assert_allclose(w, np.linalg.solve(A, b), rtol=0, atol=1e-7)
+
+
+def test_rational_form_is_the_resolvent_combination():
+    """The rational form (1 - a) I + a G + a g (g I + G)^-1 against the same
+    combination built from G itself (read off the linear form with a = 1)
+    and a dense inverse: pins the Thomas solve of g I + G (its forward and
+    backward sweeps), the field term included.  Constants are preserved and
+    diag(m d^2)(P - I) is symmetric positive semidefinite, as for any
+    rational function of G with a nonnegative symbol x (x + g - 1)/(g + x)."""
+    x, pt = _graded_case(30)
+    n = len(pt)
+    rng = np.random.default_rng(12)
+    w0 = rng.uniform(-2.0, 40.0, n)
+    for c in (0.0, 0.25):
+        G = _gs_operator(x, pt, w0, a=1.0, c=c) - np.eye(n)
+        for a, g in ((0.5, 3.0), (0.4, 2.0)):
+            P = _gs_operator(x, pt, w0, a=a, c=c, shift=g)
+            ref = (1 - a) * np.eye(n) + a * G + a * g * np.linalg.inv(g * np.eye(n) + G)
+            scale = np.abs(ref).max()
+            assert np.abs(P - ref).max() <= 1e-10 * scale, (c, a, g, np.abs(P - ref).max())
+    # on a coarse mesh G is O(1), so the resolvent part itself is checked:
+    # P - (1 - a) I - a G = a g (g I + G)^-1 to rounding
+    xc = np.linspace(0.0, 6.0, 25)
+    ptc = O.target_position(xc, O.TAU, False)[1:]
+    wc = rng.uniform(-1.0, 3.0, len(ptc))
+    nc = len(ptc)
+    Gc = _gs_operator(xc, ptc, wc, a=1.0, c=0.25) - np.eye(nc)
+    Pc = _gs_operator(xc, ptc, wc, a=0.5, c=0.25, shift=3.0)
+    res = Pc - 0.5 * np.eye(nc) - 0.5 * Gc
+    ref = 1.5 * np.linalg.inv(3.0 * np.eye(nc) + Gc)
+    assert np.abs(ref).max() > 0.1 and np.abs(Gc).max() < 1e3
+    assert np.abs(res - ref).max() <= 1e-12, np.abs(res - ref).max()
+    P = _gs_operator(x, pt, np.zeros(n), a=0.5, shift=3.0)
+    np.testing.assert_allclose(P @ np.ones(n), np.ones(n), rtol=0, atol=1e-9)
+    hl = np.diff(x)[:n]
+    hr = np.concatenate([np.diff(x)[1:n], [0.0]])
+    d = np.sqrt(np.maximum(pt, 1e-8))
+    A = np.diag(0.5 * (hl + hr) * d * d) @ (P - np.eye(n))
+    sc = np.abs(A).max()
+    assert np.abs(A - A.T).max() <= 1e-10 * sc
+    assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() >= -1e-10 * sc
