@@ -203,8 +203,11 @@ typedef struct scft_anderson_opts {
   double gs_switch_growth;/* ... or when its ||r||_inf exceeds this factor
                              times its best so far (0: never) */
   double gs_shift;        /* g > 0: the rational ground-state form
-                               g = (1 - a) f + a G f + a g (g I + G)^{-1} f,
-                             G the operator above: its symbol follows the
+                               g = (1 - a) f + a G f + a g (g I + G0)^{-1} f,
+                             G the operator above, G0 = G without its field
+                             term (so the factors of g I + G0 depend on the
+                             mesh and phi_t only and are computed once per
+                             batch, ~ms): its symbol follows the
                              inverse Debye function of the chain's response
                              to ~2% (a = 1/2, g = 3) where the linear form
                              f + a G f follows it to ~13%.  ~20% fewer

@@ -257,9 +257,10 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
      * with d = sqrt(max(phi_t, floor)) and d = 0 beyond the unknowns (the
      * wall nodes; a natural end has r_u = 0) */
     if (gs_on && o->gs_shift > 0.0) {
-      /* the rational form (1 - a) f + a G f + a g (g I + G)^{-1} f: G is
-       * tridiagonal with |lower| + |upper| = its diagonal minus c w+, so
-       * g I + G is strictly diagonally dominant and Thomas needs no pivots */
+      /* the rational form (1 - a) f + a G f + a g (g I + G0)^{-1} f, G0 = G
+       * without the field term: tridiagonal with |lower| + |upper| = its
+       * diagonal, so g I + G0 is strictly diagonally dominant and Thomas
+       * needs no pivoting; its factors depend on the mesh and phi_t only */
       const double *gs = o->gstencil;
       const double a = o->gs_precond, g = o->gs_shift;
       double *lo = (double *)malloc(sizeof(double) * 4 * (size_t)n);
@@ -275,11 +276,12 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
         t += o->gs_field_weight * fmax(w[i], 0.0) * r[i];
         wn[i] = (1.0 - a) * r[i] + a * t;
       }
-      for (int i = 0; i < n; i++) { /* forward sweep of (g I + G) y = f */
-        const double diag = g + lo[i] + up[i] + o->gs_field_weight * fmax(w[i], 0.0);
-        const double den = i > 0 ? diag - (-lo[i]) * cp[i - 1] : diag;
-        cp[i] = -up[i] / den;
-        y[i] = (i > 0 ? r[i] - (-lo[i]) * y[i - 1] : r[i]) / den;
+      for (int i = 0; i < n; i++) { /* forward sweep of (g I + G0) y = f */
+        const double diag = g + lo[i] + up[i];
+        const double den = i > 0 ? diag + lo[i] * cp[i - 1] : diag;
+        const double rinv = 1.0 / den;
+        cp[i] = -up[i] * rinv;
+        y[i] = (i > 0 ? r[i] + lo[i] * y[i - 1] : r[i]) * rinv;
       }
       for (int i = n - 2; i >= 0; i--) y[i] -= cp[i] * y[i + 1];
       for (int i = 0; i < n; i++) wn[i] += a * g * y[i];

@@ -59,8 +59,9 @@ typedef struct orc_anderson_opts {
   const double *gstencil; /* 3 n doubles (l, unused, r per unknown) of the
                              ground-state form when gs_precond > 0 */
   double gs_shift;      /* g > 0: the rational ground-state form
-                           g_f = (1 - a) f + a G f + a g (g I + G)^{-1} f,
-                           G the operator above (0: the linear form f + a G f) */
+                           g_f = (1 - a) f + a G f + a g (g I + G0)^{-1} f,
+                           G the operator above, G0 = G without its field
+                           term (0: the linear form f + a G f) */
 } orc_anderson_opts;
 
 /* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements

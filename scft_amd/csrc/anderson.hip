@@ -193,18 +193,20 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // the mesh's nodes (a natural end has no right neighbour): the operator
   // -(1/d^2)(d^2 f')' + c w+, which maps constants to themselves.  With
   // gs_shift = g > 0 the rational form
-  //   g = (1 - a_g) f + a_g G f + a_g g (g I + G)^-1 f
+  //   g = (1 - a_g) f + a_g G f + a_g g (g I + G0)^-1 f,  G0 = G with c = 0,
   // whose symbol (1 - a) + a x + a g / (g + x) follows 1/g_D(x), the
   // inverse Debye function of the chain's linear response, to ~2% for
-  // a = 1/2, g = 3 (the linear form: to ~13%); (g I + G) y = f is solved
-  // below by the Thomas recurrence, G's rows being kept for it in `tri`.
+  // a = 1/2, g = 3 (the linear form: to ~13%).  The Thomas factors of
+  // g I + G0 depend on the mesh and phi_t only (gs_pivots_kernel, once per
+  // batch): per sweep the solve is its two first-order recurrences, run as
+  // scans below.
   const double coef = gs ? a.gs_precond : a.precond;
   const bool pc = coef > 0.0 && (gs || n > 1);
   const bool rational = pc && gs && a.gs_shift > 0.0;
-  double* __restrict__ t_lo = rational ? a.tri + d.uo : nullptr;
-  double* __restrict__ t_dg = rational ? t_lo + a.tri_stride : nullptr;
-  double* __restrict__ t_up = rational ? t_dg + a.tri_stride : nullptr;
-  double* __restrict__ t_rh = rational ? t_up + a.tri_stride : nullptr;
+  const double* __restrict__ t_ri = rational ? a.tri + d.uo : nullptr;  // 1 / pivot
+  const double* __restrict__ t_al = rational ? t_ri + a.tri_stride : nullptr;  // lower / pivot
+  const double* __restrict__ t_mu = rational ? t_al + a.tri_stride : nullptr;  // upper / pivot
+  double* __restrict__ t_rh = rational ? a.tri + 3 * a.tri_stride + d.uo : nullptr;
   // (whole-wave trips: the neighbours' values come from the lanes beside,
   // and only a wave's edge lanes evaluate theirs again)
   for (int u0 = 0; u0 < n; u0 += AM_THREADS) {
@@ -239,9 +241,6 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
         double t = lo * (f - fl) + up * (f - fr);
         t += fw * f;
         if (rational) {
-          t_lo[u] = lo;
-          t_dg[u] = a.gs_shift + lo + up + fw;
-          t_up[u] = up;
           t_rh[u] = f;
           f = (1.0 - coef) * f + coef * t;
         } else {
@@ -273,53 +272,76 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   __syncthreads();
 
   if (rational) {
-    // (g I + G) y = f by Thomas: g I + G is strictly diagonally dominant, so
-    // no pivoting.  Wave 0 walks the rows in blocks of 64: each lane loads
-    // one row, every lane follows the (wave-uniform) recurrence through
-    // readlane, and the lane of a row keeps its result.  The forward sweep
-    // leaves cp in t_up and the modified right side in t_rh, the backward
-    // sweep y in t_rh.
-    if (tid < 64) {
-      const int lane = tid;
-      double cpv = 0.0, yv = 0.0;
-      for (int b0 = 0; b0 < n; b0 += 64) {
-        const int u = b0 + lane;
-        const bool ok = u < n;
-        const double Lo = ok ? t_lo[u] : 0.0, Dg = ok ? t_dg[u] : 1.0;
-        const double Up = ok ? t_up[u] : 0.0, Rh = ok ? t_rh[u] : 0.0;
-        const int cnt = min(64, n - b0);
-        double mycp = 0.0, myy = 0.0;
-        for (int j = 0; j < cnt; j++) {
-          const double lo = readlane_f64(Lo, j), dg = readlane_f64(Dg, j);
-          const double up = readlane_f64(Up, j), rh = readlane_f64(Rh, j);
-          const bool first = b0 + j == 0;
-          const double den = first ? dg : dg - (-lo) * cpv;
-          cpv = -up / den;
-          yv = (first ? rh : rh - (-lo) * yv) / den;
-          if (lane == j) {
-            mycp = cpv;
-            myy = yv;
-          }
-        }
-        if (ok) {
-          t_up[u] = mycp;
-          t_rh[u] = myy;
-        }
+    // (g I + G0) y = f with the stored factors: z_i = ri_i f_i + al_i z_{i-1},
+    // then y_i = z_i + mu_i y_{i+1}.  Both are affine recurrences with
+    // coefficients in [0, 1) (diagonal dominance), scanned in parallel: each
+    // thread composes the maps of a contiguous chunk of rows, the chunk maps
+    // are scanned across the wave (shuffles) and the waves (LDS), and each
+    // thread replays its chunk from its incoming value.
+    __shared__ double s_mapA[AM_THREADS / 64], s_mapB[AM_THREADS / 64];
+    const int lane = tid & 63, wave = tid >> 6;
+    const int C = (n + AM_THREADS - 1) / AM_THREADS;
+    const int r0 = min(n, tid * C), r1 = min(n, r0 + C);
+    // forward: chunk map z_out = A z_in + B
+    double A = 1.0, B = 0.0;
+    for (int i = r0; i < r1; i++) {
+      const double al = t_al[i];
+      B = t_ri[i] * t_rh[i] + al * B;
+      A = al * A;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {  // inclusive, lane order
+      const double Ae = __shfl_up(A, off), Be = __shfl_up(B, off);
+      if (lane >= off) {
+        B = A * Be + B;
+        A = A * Ae;
       }
-      double ynext = 0.0;
-      for (int b0 = ((n - 1) / 64) * 64; b0 >= 0; b0 -= 64) {
-        const int u = b0 + lane;
-        const bool ok = u < n;
-        const double Cp = ok ? t_up[u] : 0.0, Yv = ok ? t_rh[u] : 0.0;
-        const int cnt = min(64, n - b0);
-        double myy = Yv;
-        for (int j = cnt - 1; j >= 0; j--) {
-          double yi = readlane_f64(Yv, j);
-          if (b0 + j < n - 1) yi = yi - readlane_f64(Cp, j) * ynext;
-          ynext = yi;
-          if (lane == j) myy = yi;
-        }
-        if (ok) t_rh[u] = myy;
+    }
+    if (lane == 63) {
+      s_mapA[wave] = A;
+      s_mapB[wave] = B;
+    }
+    __syncthreads();
+    double zin = 0.0;  // z entering this wave (z_{-1} = 0)
+    for (int v = 0; v < wave; v++) zin = s_mapA[v] * zin + s_mapB[v];
+    {
+      const double Ap = __shfl_up(A, 1), Bp = __shfl_up(B, 1);
+      double z = lane > 0 ? Ap * zin + Bp : zin;
+      for (int i = r0; i < r1; i++) {
+        z = t_ri[i] * t_rh[i] + t_al[i] * z;
+        t_rh[i] = z;
+      }
+    }
+    __syncthreads();
+    // backward: chunk map y_out(first row) = A y_in(after the chunk) + B
+    A = 1.0;
+    B = 0.0;
+    for (int i = r1 - 1; i >= r0; i--) {
+      const double mu = t_mu[i];
+      B = t_rh[i] + mu * B;
+      A = mu * A;
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {  // inclusive, reverse lane order
+      const double Al = __shfl_down(A, off), Bl = __shfl_down(B, off);
+      if (lane + off < 64) {
+        B = A * Bl + B;
+        A = A * Al;
+      }
+    }
+    if (lane == 0) {
+      s_mapA[wave] = A;
+      s_mapB[wave] = B;
+    }
+    __syncthreads();
+    double yin = 0.0;  // y entering this wave from above (y_n = 0)
+    for (int v = AM_THREADS / 64 - 1; v > wave; v--) yin = s_mapA[v] * yin + s_mapB[v];
+    {
+      const double An = __shfl_down(A, 1), Bn = __shfl_down(B, 1);
+      double y = lane < 63 ? An * yin + Bn : yin;
+      for (int i = r1 - 1; i >= r0; i--) {
+        y = t_rh[i] + t_mu[i] * y;
+        t_rh[i] = y;
       }
     }
     __syncthreads();
@@ -538,6 +560,64 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
 __global__ void reset_counters_kernel(int* counters, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) counters[i] = 0;
+}
+
+// The Thomas factors of g I + G0 for the rational ground-state form, one
+// wave per instance: rows (lower l_u d_{u-1}/d_u, upper r_u d_{u+1}/d_u,
+// diagonal g + lower + upper) as in anderson_kernel; the pivot recurrence
+// den_u = diag_u - lower_u upper_{u-1} / den_{u-1} walked in blocks of 64
+// rows, each lane holding one row and every lane following the
+// (wave-uniform) recurrence through readlane.  Stores 1/den, lower/den and
+// upper/den at tri[0..2] x tri_stride.
+__global__ void __launch_bounds__(64) gs_pivots_kernel(AndersonArgs a) {
+  const int inst = blockIdx.x, lane = threadIdx.x;
+  const InstDesc d = a.desc[inst];
+  const int n = d.n;
+  const double* __restrict__ pt = a.phit + d.uo;
+  const double* __restrict__ xn = a.x + d.xo;
+  double* __restrict__ t_ri = a.tri + d.uo;
+  double* __restrict__ t_al = t_ri + a.tri_stride;
+  double* __restrict__ t_mu = t_al + a.tri_stride;
+  double cpv = 0.0;  // -upper_{u-1} / den_{u-1}
+  for (int b0 = 0; b0 < n; b0 += 64) {
+    const int u = b0 + lane;
+    double lo = 0.0, up = 0.0;
+    if (u < n) {
+      const int i = u + 1;
+      const double dv = sqrt(fmax(pt[u], a.floor));
+      const double dl = u > 0 ? sqrt(fmax(pt[u - 1], a.floor)) : 0.0;
+      const double dr = u + 1 < n ? sqrt(fmax(pt[u + 1], a.floor)) : 0.0;
+      const bool node_r = i + 1 < d.M;
+      const double hl = xn[i] - xn[i - 1];
+      const double hrr = node_r ? xn[i + 1] - xn[i] : 0.0;
+      const double mm = 0.5 * (hl + hrr);
+      const double cl = (1.0 / hl) / mm;
+      const double cr = node_r ? (1.0 / hrr) / mm : 0.0;
+      lo = cl * (dl / dv);
+      up = cr * (dr / dv);
+    }
+    const double dg = a.gs_shift + lo + up;
+    const int cnt = min(64, n - b0);
+    double myri = 0.0;
+    for (int j = 0; j < cnt; j++) {
+      const double loj = readlane_f64(lo, j), dgj = readlane_f64(dg, j);
+      const double upj = readlane_f64(up, j);
+      const double den = b0 + j == 0 ? dgj : dgj + loj * cpv;
+      const double ri = 1.0 / den;
+      cpv = -upj * ri;
+      if (lane == j) myri = ri;
+    }
+    if (u < n) {
+      t_ri[u] = myri;
+      t_al[u] = lo * myri;
+      t_mu[u] = up * myri;
+    }
+  }
+}
+
+hipError_t launch_gs_pivots(const AndersonArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(gs_pivots_kernel, dim3(a.n_inst), dim3(64), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s) {

@@ -108,7 +108,9 @@ struct scft_batch {
   // mixer
   int hist_S = 0;
   double *d_hw = nullptr, *d_hr = nullptr, *d_wcold = nullptr;
-  double* d_tri = nullptr;  // rational preconditioner rows, 4 x total_u (on demand)
+  double* d_tri = nullptr;  // rational preconditioner factors + work, 4 x total_u
+  bool tri_valid = false;    // factors computed for tri_shift / tri_floor
+  double tri_shift = 0.0, tri_floor = 0.0;
   MixState* d_ms = nullptr;
   int* d_iters = nullptr;
   int* d_mix_order = nullptr;  // instances by decreasing n (mixer dispatch order)
@@ -486,6 +488,7 @@ extern "C" int scft_batch_set_walls(scft_batch* b, const double* wall0, const do
 static int batch_set_target(scft_batch* b, const double* phit) {
   HIPCHK(hipMemcpyAsync(b->d_phit, phit, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
                         b->stream));
+  b->tri_valid = false;  // the rational preconditioner's factors depend on phi_t
   return SCFT_OK;
 }
 
@@ -864,6 +867,14 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.gs_shift = o->gs_precond > 0.0 ? o->gs_shift : 0.0;
   a.tri = b->d_tri;
   a.tri_stride = b->total_u;
+  if (a.gs_shift > 0.0 &&
+      (!b->tri_valid || b->tri_shift != a.gs_shift || b->tri_floor != a.floor)) {
+    // the factors of g I + G0: mesh and phi_t only, so once per batch
+    HIPCHK(launch_gs_pivots(a, b->stream));
+    b->tri_valid = true;
+    b->tri_shift = a.gs_shift;
+    b->tri_floor = a.floor;
+  }
   // fresh mixer state (a small kernel-free host copy of zeros is not needed:
   // the per-instance state array is rewritten from a pinned staging copy)
   if (static_cast<int>(b->ms0.size()) != b->n_inst)

@@ -93,8 +93,8 @@ struct AndersonArgs {
   int gs_switch;           // evaluations allowed in that form (0: no limit)
   double gs_switch_growth; // leave it when ||r|| > growth * best (0: never)
   double gs_shift;         // g > 0: the rational ground-state form (0: linear)
-  double* tri;             // 4 x tri_stride doubles: the rational form's rows
-  long long tri_stride;    // (lower, diagonal, upper, right side), at desc.uo
+  double* tri;             // 4 x tri_stride doubles at desc.uo: the factors of
+  long long tri_stride;    // g I + G0 (1/pivot, lower/pivot, upper/pivot), work
 };
 
 // Size bucket of an instance with n unknowns: the power of two >= n (at
@@ -121,6 +121,9 @@ bool march_variant_exists(int K, int W);
 hipError_t launch_irk4(int K, int W, const MarchArgs& a, int grid, hipStream_t s);
 int irk4_max_grid(int K, int W, int half_max);  // 0: the workgroup does not fit
 bool irk4_variant_exists(int K, int W);
+// The Thomas factors of the rational preconditioner's g I + G0 (once per
+// batch and shift).
+hipError_t launch_gs_pivots(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_anderson(const AndersonArgs& a, hipStream_t s);
 hipError_t launch_reset(int* counters, int n, hipStream_t s);
 constexpr int MIX_MAX_HISTORY = 16;

@@ -294,21 +294,22 @@ def test_ground_state_form_keeps_finite_termination(n):
 
 
 def test_rational_form_is_the_resolvent_combination():
-    """The rational form (1 - a) I + a G + a g (g I + G)^-1 against the same
-    combination built from G itself (read off the linear form with a = 1)
-    and a dense inverse: pins the Thomas solve of g I + G (its forward and
-    backward sweeps), the field term included.  Constants are preserved and
+    """The rational form (1 - a) I + a G + a g (g I + G0)^-1 (G0: G without
+    the field term) against the same combination built from G and G0
+    themselves (read off the linear form with a = 1) and a dense inverse:
+    pins the Thomas solve of g I + G0 (its forward and backward sweeps).  Constants are preserved and
     diag(m d^2)(P - I) is symmetric positive semidefinite, as for any
     rational function of G with a nonnegative symbol x (x + g - 1)/(g + x)."""
     x, pt = _graded_case(30)
     n = len(pt)
     rng = np.random.default_rng(12)
     w0 = rng.uniform(-2.0, 40.0, n)
+    G0 = _gs_operator(x, pt, w0, a=1.0, c=0.0) - np.eye(n)
     for c in (0.0, 0.25):
         G = _gs_operator(x, pt, w0, a=1.0, c=c) - np.eye(n)
         for a, g in ((0.5, 3.0), (0.4, 2.0)):
             P = _gs_operator(x, pt, w0, a=a, c=c, shift=g)
-            ref = (1 - a) * np.eye(n) + a * G + a * g * np.linalg.inv(g * np.eye(n) + G)
+            ref = (1 - a) * np.eye(n) + a * G + a * g * np.linalg.inv(g * np.eye(n) + G0)
             scale = np.abs(ref).max()
             assert np.abs(P - ref).max() <= 1e-10 * scale, (c, a, g, np.abs(P - ref).max())
     # on a coarse mesh G is O(1), so the resolvent part itself is checked:
@@ -318,9 +319,10 @@ def test_rational_form_is_the_resolvent_combination():
     wc = rng.uniform(-1.0, 3.0, len(ptc))
     nc = len(ptc)
     Gc = _gs_operator(xc, ptc, wc, a=1.0, c=0.25) - np.eye(nc)
+    G0c = _gs_operator(xc, ptc, wc, a=1.0, c=0.0) - np.eye(nc)
     Pc = _gs_operator(xc, ptc, wc, a=0.5, c=0.25, shift=3.0)
     res = Pc - 0.5 * np.eye(nc) - 0.5 * Gc
-    ref = 1.5 * np.linalg.inv(3.0 * np.eye(nc) + Gc)
+    ref = 1.5 * np.linalg.inv(3.0 * np.eye(nc) + G0c)
     assert np.abs(ref).max() > 0.1 and np.abs(Gc).max() < 1e3
     assert np.abs(res - ref).max() <= 1e-12, np.abs(res - ref).max()
     P = _gs_operator(x, pt, np.zeros(n), a=0.5, shift=3.0)
