@@ -87,35 +87,31 @@ WORKLOADS = {
 }
 
 
-# The ground-state form of the preconditioner, used by every leg:
-# g_u = f_u + 0.45 [ l (d_{u-1}/d_u)(f_u - f_{u-1}) + r (d_{u+1}/d_u)(f_u - f_{u+1})
-#                    + 0.25 max(w_u, 0) f_u ],  d = sqrt(phi_t),
-# the operator -(1/d^2)(d^2 f')' + w+/4.  On production-mix instances the
-# preconditioned Jacobian has every eigenvalue within 0.14 of 1 with it.
-# Settings from whole-batch solves (production mix, seconds per solve from
-# w0 / from predictions): field weight 0, 0.25, 0.5, 0.75, 1 -> 1.17, 1.06,
-# 1.25, 1.46, 2.32 s from w0 (weight 1 leaves a tail to 110 evaluations),
-# 0.35-0.36 s from predictions at any weight; coefficient 0.4, 0.45, 0.5,
-# 0.55 (weight 0.25 / 0.5) -> 1.18, 1.05, 1.06, 2.67 s from w0, and 0.337
-# against 0.360 s from predictions at 0.45 and 0.5.  The 4096-instance mix
-# agrees (0.102-0.110 s from w0, 0.047-0.049 s from predictions).
-GS_PRECOND = 0.45
+# The ground-state operator G = -(1/d^2)(d^2 f')' + c w+ (d = sqrt(phi_t),
+# c = 1/4) preconditions every leg, in its rational form
+#   (1 - a) f + a G f + a g (g I + G0)^-1 f,  a = 1/2, g = 3
+# (G0: G without the field term), whose symbol follows the inverse Debye
+# function 1/g_D(x) = x^2 / (2 (e^-x + x - 1)) to ~2.5%: the preconditioned
+# Jacobian's eigenvalues lie within ~0.03 of 1 near the solution.  From w0
+# the first updates are far from it: there the linear form f + a G f is
+# used until an instance's ||r||_inf first falls below 0.5, then the
+# rational one with the history cleared.
+# Whole production-mix batches, seconds per solve: from w0, linear form
+# throughout (a = 0.45) 1.058, rational after ||r|| < 0.5 / 0.3 / 0.1:
+# 0.769 / 0.785 / 0.805 (a = 0.45: 0.872); from predictions, rational from
+# the first update 0.286 against 0.339 linear (g = 2 or 4: 0.30-0.31).
+# Field weight c from w0 (linear form): 0, 0.25, 0.5, 1 -> 1.17, 1.06, 1.25,
+# 2.32 s (1 leaves a tail to 110 evaluations).
+GS_PRECOND = 0.5
+GS_SHIFT = 3.0
 GS_FIELD_WEIGHT = 0.25
+COLD_SHIFT_BELOW = 0.5
 # Predicted starts: an instance not converged after 24 evaluations goes over
 # to the plain form with its current field (history cleared), a safety net
 # that no instance of either bench mix reaches.
 GS_SWITCH = 24
 # Predicted starts need a shorter history than w0: 8 columns.
 WARM_HISTORY = 8
-# ... and take the rational form (1 - a) f + a G f + a g (g I + G)^-1 f with
-# the inverse Debye function's coefficients a = 1/2, g = 3: its symbol follows
-# 1/g_D(x) = x^2 / (2 (e^-x + x - 1)) to ~2% (the linear form to ~8% at best),
-# which leaves the preconditioned Jacobian's eigenvalues within ~0.03 of 1.
-# Production mix from predictions: 0.286 s (mean 2.92 evaluations) against
-# 0.339 s (3.51) for the linear form; g = 2 or 4: 0.30-0.31 s; a = 0.45:
-# 0.335 s.  From w0 it does not help (CPU oracle: 10.2 against 9.7).
-WARM_GS_PRECOND = 0.5
-WARM_GS_SHIFT = 3.0
 # The network: 40 000 converged fields, Adam in four stages of 100 epochs
 # with the rate stepped down 1e-3 -> 3e-5.  A single 50-epoch stage at 1e-3
 # leaves networks whose warm solves vary with the shuffling seed (production
@@ -125,17 +121,29 @@ TRAIN_SCHEDULE = ((100, 1e-3), (100, 3e-4), (100, 1e-4), (100, 3e-5))
 
 
 def solver_settings(workload):
+    """The solver from w0 (the cold leg, the training solves, the CPU
+    baseline's cold pair); predicted starts change it in warm_overrides."""
     return dict(history=16, beta=1.0, tol=1e-6, step_cap=0.0, floor=1e-8,
                 precond=WORKLOADS[workload]["precond"], backtrack=1e30, max_halvings=30,
-                field_floor=0.5, gs_precond=GS_PRECOND, gs_field_weight=GS_FIELD_WEIGHT)
+                field_floor=0.5, gs_precond=GS_PRECOND, gs_field_weight=GS_FIELD_WEIGHT,
+                gs_shift=GS_SHIFT, gs_shift_below=COLD_SHIFT_BELOW)
+
+
+def warm_overrides():
+    """From a predicted start: a shorter history and the plain form after
+    GS_SWITCH evaluations.  The threshold rule stays: a prediction's first
+    residual is far below it, so the rational form applies from the first
+    update, while an instance that falls back to w0 starts linear as the
+    cold leg does."""
+    return dict(history=WARM_HISTORY, gs_switch=GS_SWITCH)
 
 
 def solver_text(workload):
     return (f"device Anderson, history 16 (cold) or {WARM_HISTORY} (predicted starts), log "
-            f"mixing preconditioned by the ground-state operator G = -(1/d^2)(d^2 f')' + "
-            f"{GS_FIELD_WEIGHT} w+, d = sqrt(phi_t): I + {GS_PRECOND} G from w0, "
-            f"(1 - {WARM_GS_PRECOND}) I + {WARM_GS_PRECOND} G + {WARM_GS_PRECOND}*"
-            f"{WARM_GS_SHIFT} ({WARM_GS_SHIFT} I + G)^-1 from predictions (plain form I + "
+            f"mixing preconditioned by (1 - {GS_PRECOND}) I + {GS_PRECOND} G + "
+            f"{GS_PRECOND}*{GS_SHIFT} ({GS_SHIFT} I + G0)^-1, G = -(1/d^2)(d^2 f')' + "
+            f"{GS_FIELD_WEIGHT} w+, d = sqrt(phi_t), G0 = G without w+ (from w0: I + "
+            f"{GS_PRECOND} G until ||r|| < {COLD_SHIFT_BELOW}; predicted starts: plain form I + "
             f"{WORKLOADS[workload]['precond']} M^-1 K after {GS_SWITCH} evaluations), beta 1.0, "
             "no step cap, NaN backtracking, field bounded below by -0.5/dt")
 
@@ -228,12 +236,8 @@ def warm_solves(S, b, cfg, w0, args, ws, rec):
     prep = WS.Prepared(S, cfg)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
-    # from a predicted start: the rational ground-state form with its switch
-    # to the plain form, and a shorter history
-    wopts.gs_switch = GS_SWITCH
-    wopts.history = WARM_HISTORY
-    wopts.gs_precond = WARM_GS_PRECOND
-    wopts.gs_shift = WARM_GS_SHIFT
+    for k, v in warm_overrides().items():  # from a predicted start
+        setattr(wopts, k, v)
     w0_flat = b.join(w0)
     b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
     plan = ws.device_plan(b, prep)  # network inputs and mapping tables, resident in HBM
@@ -317,10 +321,8 @@ def continuation_compare(args):
     opts = solver_opts(S, args)
     wopts = solver_opts(S, args)
     wopts.fallback_factor = 10.0
-    wopts.gs_switch = GS_SWITCH
-    wopts.history = WARM_HISTORY
-    wopts.gs_precond = WARM_GS_PRECOND
-    wopts.gs_shift = WARM_GS_SHIFT
+    for k, v in warm_overrides().items():  # a transferred field is a predicted start
+        setattr(wopts, k, v)
     w0_flat = fine.join(w0)
     # the spline transfer: coarse unknowns (with the wall node) -> fine unknowns
     src_x = [x for x in xc]
@@ -399,6 +401,56 @@ def stepper_compare(args):
         b.close()
         log(f"bench: {'IE' if stepper == S.STEP_IE else 'IRK4'} N={N}: {ms:.3f} ms per sweep of "
             f"{n} instances, ||r|| of the reference field {out['rnorm'][0]:.2e}")
+
+
+def gap_curve(args):
+    """First residual -> evaluations with the warm leg's solver, on the bench
+    batch itself (device only): every instance converged from w0 to 1e-11,
+    its field perturbed by a smooth relative bump of size eps,
+    w (1 + eps (cos(pi x/L) + sin(3 pi x/L)/2)), and solved back to the
+    bench tolerance from there (no fallback).  Shows how good a prediction
+    has to be for a given number of evaluations.  One JSON line."""
+    import scft_amd as S
+    from scft_amd import workloads as WL
+    if args.n_inst <= 0:
+        args.n_inst = WORKLOADS[args.workload]["n_inst"]
+    cfg = make_workload(WL, args.workload, args.n_inst)
+    w0 = [w[1:] for w in cfg["w0"]]
+    b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN)
+    b.upload(w0)
+    b.autotune(reps=2)
+    tight = solver_opts(S, args)
+    tight.tol = 1e-11
+    b.upload(w0)
+    b.solve_resident(tight)
+    ref = b.download()
+    log(f"bench: gap curve: {ref['converged']}/{args.n_inst} converged to 1e-11")
+    wopts = solver_opts(S, args)
+    for k, v in warm_overrides().items():
+        setattr(wopts, k, v)
+    bumps = [np.cos(np.pi * x[1:] / x[-1]) + 0.5 * np.sin(3.0 * np.pi * x[1:] / x[-1])
+             for x in cfg["xs"]]
+    rows = []
+    for eps in (1e-2, 1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 3e-6, 1e-6):
+        start = [w * (1.0 + eps * bb) for w, bb in zip(ref["w"], bumps)]
+        first = b.eval(start)["rnorm"]
+        ts = []
+        for _ in range(2):
+            b.upload(start)
+            ts.append(b.solve_resident(wopts)["seconds"])
+        res = b.download()
+        it = np.asarray(res["iters"])
+        row = {"eps": eps,
+               "first_residual_q10_q50_q90": [float(f"{v:.3g}") for v in
+                                               np.quantile(first, [0.1, 0.5, 0.9])],
+               "mean_evaluations": round(float(it.mean()), 2), "max_evaluations": int(it.max()),
+               "solve_seconds": round(min(ts), 4), "converged": int(res["converged"])}
+        log(f"bench: gap curve {row}")
+        rows.append(row)
+    b.close()
+    print(json.dumps({"workload": WORKLOADS[args.workload]["text"].format(n=args.n_inst),
+                      "solver": solver_text(args.workload) + " (predicted-start settings)",
+                      "curve": rows}), flush=True)
 
 
 def cpu_baseline(cfg, gpu_iters, args, threads, budget_s, starts=None, overrides=None):
@@ -522,6 +574,10 @@ def main():
                     help="the reference's mesh continuation as the warm start of the bench "
                          "batch (coarse solve + spline transfer + solve), against the cold "
                          "start (one JSON line, not the bench line)")
+    ap.add_argument("--gap-curve", action="store_true",
+                    help="first residual -> evaluations of the warm leg's solver from "
+                         "perturbed converged fields of the bench batch (one JSON line, not "
+                         "the bench line)")
     ap.add_argument("--stepper-compare", action="store_true",
                     help="time one sweep of 4096 copies of the reference's 33-node deal.II "
                          "problem with implicit Euler and with IRK4 (no JSON line)")
@@ -536,6 +592,9 @@ def main():
         return
     if args.continuation_compare:
         continuation_compare(args)
+        return
+    if args.gap_curve:
+        gap_curve(args)
         return
 
     if args.copy_calibration:
@@ -677,9 +736,7 @@ def main():
         if warm:  # the measured workload: the same predicted fields, the warm leg's options
             cb = cpu_baseline(cfg, res["iters"], args, threads, args.cpu_budget,
                               starts=warm["starts"],
-                              overrides=dict(history=WARM_HISTORY, gs_switch=GS_SWITCH,
-                                             gs_precond=WARM_GS_PRECOND,
-                                             gs_shift=WARM_GS_SHIFT))
+                              overrides=warm_overrides())
             what = "the network's predicted fields (no fallback)"
         else:
             cb, what = cbc, "w0"

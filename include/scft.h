@@ -54,6 +54,7 @@ extern "C" {
 #define SCFT_ST_FALLBACK 16u /* warm start rejected, restarted cold */
 #define SCFT_ST_FAILED 32u   /* evaluation failed (NaN / pivot) beyond recovery */
 #define SCFT_ST_PLAIN_FORM 64u /* left the ground-state form (gs_switch) */
+#define SCFT_ST_RATIONAL 128u  /* reached the rational form (gs_shift_below) */
 
 /* ---- library lifetime ---------------------------------------------------
  * Replaces PetscInitialize/PetscFinalize around the evaluator
@@ -213,6 +214,12 @@ typedef struct scft_anderson_opts {
                              f + a G f follows it to ~13%.  ~20% fewer
                              evaluations from predicted starts, none fewer
                              from w0 (0: the linear form) */
+  double gs_shift_below;  /* > 0: the linear form until an instance's
+                             ||r||_inf first falls below this, then the
+                             rational one (history cleared, status
+                             SCFT_ST_RATIONAL); from w0 with 0.5: ~25% fewer
+                             evaluations than the linear form throughout.
+                             0: the rational form from the first update */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

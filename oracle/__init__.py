@@ -63,7 +63,7 @@ class AndersonOpts(C.Structure):
                 ("field_floor", C.c_double), ("field_lower", C.c_double),
                 ("gs_precond", C.c_double), ("gs_field_weight", C.c_double),
                 ("gs_switch", C.c_int), ("gs_switch_growth", C.c_double),
-                ("gstencil", C.c_void_p), ("gs_shift", C.c_double)]
+                ("gstencil", C.c_void_p), ("gs_shift", C.c_double), ("gs_shift_below", C.c_double)]
 
 
 class AndersonStats(C.Structure):
@@ -321,11 +321,11 @@ def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=2000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
                   precond=0.0, backtrack=0.0, max_halvings=30, field_floor=0.0, field_lower=0.0,
                   gs_precond=0.0, gs_field_weight=0.25, gs_switch=0, gs_switch_growth=0.0,
-                  gs_shift=0.0):
+                  gs_shift=0.0, gs_shift_below=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, precond, None, backtrack, max_halvings, field_floor,
                         field_lower, gs_precond, gs_field_weight, gs_switch,
-                        gs_switch_growth, None, gs_shift)
+                        gs_switch_growth, None, gs_shift, gs_shift_below)
 
 
 def solve_anderson(x, phi_t, N, w_full, opts=None, bc=BC_DN, cmat=C_ROWSCALED, trace_len=0):

@@ -208,6 +208,7 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
   int stored = 0, head = 0, k, rc = 1, restarts = 0, halvings = 0;
   double best = INFINITY, err = INFINITY, eacc = INFINITY;
   int gs_on = o->gs_precond > 0.0;
+  int rat_on = o->gs_shift_below > 0.0 ? 0 : 1;
   for (k = 0; k < o->max_iter; k++) {
     int ferr = F(n, w, r, ctx);
     err = 0.0;
@@ -242,6 +243,11 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
       restarts++;
     }
     if (err < best) best = err;
+    if (gs_on && o->gs_shift > 0.0 && !rat_on && err < o->gs_shift_below) {
+      rat_on = 1; /* near the solution: the rational form, history cleared */
+      stored = 0;
+      restarts++;
+    }
     /* mixing residual: r itself, r/max(phi_t, floor), or log(phi/phi_t) */
     if (o->transform != ORC_MIX_PLAIN && phi_t_unk) {
       for (int i = 0; i < n; i++) {
@@ -256,7 +262,7 @@ int orc_anderson(double *w, int n, orc_residual_fn F, void *ctx,
      *                   + r_u (d_{u+1}/d_u)(f_u - f_{u+1}) + c max(w_u,0) f_u ]
      * with d = sqrt(max(phi_t, floor)) and d = 0 beyond the unknowns (the
      * wall nodes; a natural end has r_u = 0) */
-    if (gs_on && o->gs_shift > 0.0) {
+    if (gs_on && o->gs_shift > 0.0 && rat_on) {
       /* the rational form (1 - a) f + a G f + a g (g I + G0)^{-1} f, G0 = G
        * without the field term: tridiagonal with |lower| + |upper| = its
        * diagonal, so g I + G0 is strictly diagonally dominant and Thomas

@@ -62,6 +62,9 @@ typedef struct orc_anderson_opts {
                            g_f = (1 - a) f + a G f + a g (g I + G0)^{-1} f,
                            G the operator above, G0 = G without its field
                            term (0: the linear form f + a G f) */
+  double gs_shift_below; /* > 0: the linear form until ||r||_inf first falls
+                           below this, then the rational one (history cleared
+                           at the change); 0: the rational form throughout */
 } orc_anderson_opts;
 
 /* Stencil of M_L^{-1} K on the grid of unknowns alone (linear elements

@@ -29,6 +29,7 @@ C_ROWSCALED, C_CONSISTENT = 0, 1
 STEP_IE, STEP_IRK4 = 0, 1
 ST_CONVERGED, ST_NAN, ST_PIVOT, ST_MAXITER, ST_FALLBACK, ST_FAILED = 1, 2, 4, 8, 16, 32
 ST_PLAIN_FORM = 64  # left the ground-state form of the preconditioner
+ST_RATIONAL = 128   # reached the rational form's threshold (gs_shift_below)
 MIX_PLAIN, MIX_RELATIVE, MIX_LOG = 0, 1, 2
 L_REF = 3.72374357332160
 TAU = 0.530252230020752
@@ -67,7 +68,8 @@ class AndersonOpts(C.Structure):
                 ("backtrack", C.c_double), ("max_halvings", C.c_int),
                 ("field_floor", C.c_double), ("gs_precond", C.c_double),
                 ("gs_field_weight", C.c_double), ("gs_switch", C.c_int),
-                ("gs_switch_growth", C.c_double), ("gs_shift", C.c_double)]
+                ("gs_switch_growth", C.c_double), ("gs_shift", C.c_double),
+                ("gs_shift_below", C.c_double)]
 
 
 class BroydenStats(C.Structure):
@@ -219,11 +221,13 @@ def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   restart_factor=1e30, regularization=1e-14, transform=MIX_PLAIN, floor=1e-8,
                   fallback_factor=0.0, check_every=8, progress_seconds=0.0, precond=0.0,
                   backtrack=0.0, max_halvings=30, field_floor=0.0, gs_precond=0.0,
-                  gs_field_weight=0.25, gs_switch=0, gs_switch_growth=0.0, gs_shift=0.0):
+                  gs_field_weight=0.25, gs_switch=0, gs_switch_growth=0.0, gs_shift=0.0,
+                  gs_shift_below=0.0):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, fallback_factor, check_every, progress_seconds,
                         precond, backtrack, max_halvings, field_floor, gs_precond,
-                        gs_field_weight, gs_switch, gs_switch_growth, gs_shift)
+                        gs_field_weight, gs_switch, gs_switch_growth, gs_shift,
+                        gs_shift_below)
 
 
 class Batch:

@@ -169,13 +169,24 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // gs_switch evaluations, or when its error grows past gs_switch_growth
   // times the best so far
   bool gs = a.gs_precond > 0.0 && !(st0 & ST_PLAIN_FORM);
+  unsigned st_new = st0;
   if (gs && ((a.gs_switch > 0 && k > a.gs_switch) ||
              (a.gs_switch_growth > 0.0 && err > a.gs_switch_growth * ms.best))) {
     gs = false;
     stored = 0;
     ms.restarts++;
-    if (tid == 0) a.status[inst] = st0 | ST_PLAIN_FORM;
+    st_new |= ST_PLAIN_FORM;
   }
+  // the rational form (below) from the start, or, with gs_shift_below, once
+  // the error first falls below that threshold (history cleared then)
+  bool rat_ok = a.gs_shift_below <= 0.0 || (st0 & ST_RATIONAL);
+  if (gs && a.gs_shift > 0.0 && !rat_ok && err < a.gs_shift_below) {
+    rat_ok = true;
+    stored = 0;
+    ms.restarts++;
+    st_new |= ST_RATIONAL;
+  }
+  if (tid == 0 && st_new != st0) a.status[inst] = st_new;
   if (err < ms.best) ms.best = err;
   const int cur = __builtin_amdgcn_readfirstlane(ms.head);
   int m = __builtin_amdgcn_readfirstlane(stored < a.history ? stored : a.history);
@@ -202,7 +213,7 @@ __global__ void __launch_bounds__(AM_THREADS) anderson_kernel(AndersonArgs a) {
   // scans below.
   const double coef = gs ? a.gs_precond : a.precond;
   const bool pc = coef > 0.0 && (gs || n > 1);
-  const bool rational = pc && gs && a.gs_shift > 0.0;
+  const bool rational = pc && gs && a.gs_shift > 0.0 && rat_ok;
   const double* __restrict__ t_ri = rational ? a.tri + d.uo : nullptr;  // 1 / pivot
   const double* __restrict__ t_al = rational ? t_ri + a.tri_stride : nullptr;  // lower / pivot
   const double* __restrict__ t_mu = rational ? t_al + a.tri_stride : nullptr;  // upper / pivot

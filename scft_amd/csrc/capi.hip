@@ -764,6 +764,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->gs_switch = 0;
   o->gs_switch_growth = 0.0;
   o->gs_shift = 0.0;
+  o->gs_shift_below = 0.0;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -820,7 +821,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   if (!(o->precond >= 0.0) || !(o->backtrack >= 0.0) || o->max_halvings < 0) return SCFT_E_ARG;
   if (!(o->field_floor >= 0.0 && o->field_floor < 1.0)) return SCFT_E_ARG;
   if (!(o->gs_precond >= 0.0) || !(o->gs_field_weight >= 0.0) || o->gs_switch < 0 ||
-      !(o->gs_switch_growth >= 0.0) || !(o->gs_shift >= 0.0))
+      !(o->gs_switch_growth >= 0.0) || !(o->gs_shift >= 0.0) || !(o->gs_shift_below >= 0.0))
     return SCFT_E_ARG;
   SCFT_TRY(ensure_mixer(b, o->history + 1));
   if (o->gs_shift > 0.0 && o->gs_precond > 0.0 && !b->d_tri) {
@@ -865,6 +866,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   a.gs_switch = o->gs_switch;
   a.gs_switch_growth = o->gs_switch_growth;
   a.gs_shift = o->gs_precond > 0.0 ? o->gs_shift : 0.0;
+  a.gs_shift_below = o->gs_shift_below;
   a.tri = b->d_tri;
   a.tri_stride = b->total_u;
   if (a.gs_shift > 0.0 &&

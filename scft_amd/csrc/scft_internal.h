@@ -48,6 +48,7 @@ constexpr unsigned ST_MAXITER = 8u;
 constexpr unsigned ST_FALLBACK = 16u;
 constexpr unsigned ST_FAILED = 32u;
 constexpr unsigned ST_PLAIN_FORM = 64u;  // left the ground-state form
+constexpr unsigned ST_RATIONAL = 128u;   // reached the rational form's threshold
 // instances the march skips: converged, out of iterations, or given up by the
 // mixer (FAILED without a NaN: growth with no halvings left)
 constexpr unsigned ST_DONE_MASK = ST_CONVERGED | ST_NAN | ST_PIVOT | ST_MAXITER | ST_FAILED;
@@ -93,6 +94,7 @@ struct AndersonArgs {
   int gs_switch;           // evaluations allowed in that form (0: no limit)
   double gs_switch_growth; // leave it when ||r|| > growth * best (0: never)
   double gs_shift;         // g > 0: the rational ground-state form (0: linear)
+  double gs_shift_below;   // > 0: rational only once ||r|| < this (linear before)
   double* tri;             // 4 x tri_stride doubles at desc.uo: the factors of
   long long tri_stride;    // g I + G0 (1/pivot, lower/pivot, upper/pivot), work
 };

@@ -109,11 +109,13 @@ def test_field_floor_trajectory_matches_oracle():
             assert dw <= 1e-8
 
 
-@pytest.mark.parametrize("K,switch,start,shift",
-                         [(2, 0, "near", 0.0), (6, 0, "near", 0.0), (6, 3, "near", 0.0),
-                          (3, 0, "cold", 0.0), (2, 0, "near", 3.0), (5, 0, "near", 3.0),
-                          (5, 2, "near", 3.0), (3, 0, "cold", 3.0)])
-def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start, shift):
+@pytest.mark.parametrize("K,switch,start,shift,below",
+                         [(2, 0, "near", 0.0, 0.0), (6, 0, "near", 0.0, 0.0),
+                          (6, 3, "near", 0.0, 0.0), (3, 0, "cold", 0.0, 0.0),
+                          (2, 0, "near", 3.0, 0.0), (5, 0, "near", 3.0, 0.0),
+                          (5, 2, "near", 3.0, 0.0), (3, 0, "cold", 3.0, 0.0),
+                          (6, 0, "cold", 3.0, 0.5)])
+def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start, shift, below):
     """The ground-state form of the preconditioner (anderson.hip) against
     orc_anderson's, over K updates, on a stiff graded instance and two
     uniform ones (bar 1e-8 relative): from starts near the solution (the
@@ -121,7 +123,8 @@ def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start, shift):
     from w0; with gs_switch = 3 both leave the form for the plain one after
     the third evaluation (the second with the rational form, which reaches
     the tolerance by the third); shift = 3 is the rational form (its Thomas
-    solve on the device against the oracle's).  From w0 the comparison stops
+    solve on the device against the oracle's), from the first update or,
+    with below = 0.5, once ||r|| falls below 0.5 (linear form before).  From w0 the comparison stops
     after 3 updates: on
     the graded instance the fourth evaluation's residual is 2.9 and the
     fifth's 1.3e8 (the oracle's own trajectory), an excursion through which
@@ -148,10 +151,13 @@ def test_gs_preconditioner_trajectory_matches_oracle(K, switch, start, shift):
         starts.append(w)
     o_dev, o_cpu = _opts_pair(history=16, transform=2, beta=1.0, step_cap=0.0, max_iter=K + 1,
                               precond=0.5, field_floor=0.5, gs_precond=0.5,
-                              gs_field_weight=1.0, gs_switch=switch, gs_shift=shift)
+                              gs_field_weight=1.0, gs_switch=switch, gs_shift=shift,
+                              gs_shift_below=below)
     o_cpu.max_iter = K
     b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
     out = b.solve([w[1:] for w in starts], o_dev)
+    if below:  # every instance reached the threshold within the K updates
+        assert ((out["status"] & S.ST_RATIONAL) != 0).all()
     if switch:  # every instance left the form, or converged before it had to
         left = (out["status"] & S.ST_PLAIN_FORM) != 0
         early = ((out["status"] & S.ST_CONVERGED) != 0) & (out["iters"] <= switch)

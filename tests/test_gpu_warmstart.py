@@ -32,15 +32,16 @@ def test_warm_start_on_held_out_instances_as_specified():
     1e-3, batch 256, 50 epochs; then 2048 instances from disjoint seeds solved
     from w0 and from the prediction (device plan, outliers back to w0), with
     the evaluation counts of both, each with the ground-state preconditioner
-    (the linear form from w0; from the predictions the rational form,
-    history 8 and the plain form after 24 evaluations).  Both solves converge every instance to the same
+    (from w0 the linear form until ||r|| < 0.5, then the rational one; from
+    the predictions the rational form from the first update, history 8 and
+    the plain form after 24 evaluations).  Both solves converge every instance to the same
     solution."""
     import scft_amd as S
     from scft_amd import warmstart as WS, workloads as WL
     kw = dict(beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0, transform=S.MIX_LOG,
               precond=0.5, backtrack=1e30, check_every=4, floor=1e-8, field_floor=0.5,
-              gs_field_weight=0.25)
-    opts = S.anderson_opts(history=16, gs_precond=0.45, **kw)
+              gs_field_weight=0.25, gs_precond=0.5, gs_shift=3.0)
+    opts = S.anderson_opts(history=16, gs_shift_below=0.5, **kw)
     X, Y = _train_fields(S, WS, WL, opts, 20000, 4242)
     ws = WS.WarmStart(S, seed=5)
     hist = ws.fit(X, Y, epochs=50, batch=256, lr=1e-3, seed=6)
@@ -58,8 +59,7 @@ def test_warm_start_on_held_out_instances_as_specified():
     prep = WS.Prepared(S, test)
     plan = ws.device_plan(b, prep)
     plan.predict()
-    wopts = S.anderson_opts(history=8, fallback_factor=10.0, gs_switch=24, gs_precond=0.5,
-                            gs_shift=3.0, **kw)
+    wopts = S.anderson_opts(history=8, fallback_factor=10.0, gs_switch=24, **kw)
     b.solve_resident(wopts)
     warm = b.download()
     plan.close()

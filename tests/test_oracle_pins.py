@@ -334,3 +334,53 @@ def test_rational_form_is_the_resolvent_combination():
     sc = np.abs(A).max()
     assert np.abs(A - A.T).max() <= 1e-10 * sc
     assert np.linalg.eigvalsh(0.5 * (A + A.T)).min() >= -1e-10 * sc
+
+
+def test_rational_form_threshold_switch():
+    """gs_shift_below: the linear form until ||r|| first falls below the
+    threshold, then the rational one with the history cleared.  A threshold
+    below every residual gives the linear form's iterates bit for bit, one
+    above the first residual the rational form's (the switch at the first
+    evaluation clears an empty history), and one in between neither."""
+    n = 12
+    rng = np.random.default_rng(77)
+    Q = np.linalg.qr(rng.standard_normal((n, n)))[0]
+    A = Q @ np.diag(np.geomspace(0.3, 2.0, n)) @ Q.T
+    b = rng.standard_normal(n)
+    x = np.linspace(0.0, 1.0, n + 1)
+    gs = np.zeros(3 * n)
+    O.lib().orc_gs_stencil(np.ascontiguousarray(x), n + 1, O.BC_DN, gs)
+    phi_t = np.geomspace(1e-3, 1.0, n)
+
+    def run(**kw):
+        calls = []
+
+        def F(nn, w, r, ctx):
+            wv = np.ctypeslib.as_array(w, shape=(nn,))
+            rv = b - A @ wv
+            for i in range(nn):
+                r[i] = rv[i]
+            calls.append(float(np.abs(rv).max()))
+            return 0
+
+        cb = O.RESIDUAL_FN(F)
+        opts = O.anderson_opts(history=4, beta=1.0, tol=0.0, max_iter=6, step_cap=0.0,
+                               regularization=0.0, transform=O.MIX_PLAIN, gs_precond=0.02,
+                               gs_field_weight=0.0, **kw)
+        opts.gstencil = gs.ctypes.data
+        w = np.zeros(n)
+        stt = O.AndersonStats(0, 0, 0.0, None, 0)
+        O.lib().orc_anderson(w, n, C.cast(cb, C.c_void_p), None, C.byref(opts),
+                             np.ascontiguousarray(phi_t).ctypes.data, C.byref(stt))
+        return w, calls
+
+    lin, calls = run()
+    rat, _ = run(gs_shift=3.0)
+    assert not np.array_equal(lin, rat)
+    low, _ = run(gs_shift=3.0, gs_shift_below=1e-12)
+    high, _ = run(gs_shift=3.0, gs_shift_below=10.0 * max(calls))
+    assert np.array_equal(low, lin) and np.array_equal(high, rat)
+    # a threshold the first residual is above and a later one below
+    assert min(calls[1:]) < 0.5 * calls[0]
+    mid, _ = run(gs_shift=3.0, gs_shift_below=np.sqrt(calls[0] * min(calls[1:])))
+    assert not np.array_equal(mid, lin) and not np.array_equal(mid, rat)
