@@ -134,16 +134,20 @@ def dataset(S, cfg: dict, w_solved: list, converged: np.ndarray):
     return X.astype(np.float32), Y.astype(np.float32)
 
 
-def normalisation(X: np.ndarray, Y: np.ndarray):
+def normalisation(X: np.ndarray, Y: np.ndarray, node_weight: float = 1.0):
     """Per-feature mean and std for the three scalars and the outputs; the
     sampled cold-start field keeps unit scale (its deterministic part is the
     same profile in every instance, so what varies is the 1e-3 noise, which
-    a per-feature std would blow up to unit size)."""
+    a per-feature std would blow up to unit size).  The node samples next to
+    the wall get their scale divided by `node_weight`, which weights their
+    errors by node_weight^2 in the training loss: the first residual of a
+    prediction on a graded mesh is set mostly there."""
     xm, xs = X.mean(axis=0), X.std(axis=0)
     ym, ys = Y.mean(axis=0), Y.std(axis=0)
     xs[N_SCALARS:] = 1.0
     xs[xs < 1e-6] = 1.0
     ys[ys < 1e-6] = 1.0
+    ys[GRID:] /= node_weight
     return xm, xs, ym, ys
 
 
@@ -174,12 +178,14 @@ class WarmStart:
         self.S = S
         self.mlp = S.Mlp(DIMS, seed=seed)
 
-    def fit(self, X, Y, epochs=50, batch=256, lr=1e-3, seed=2, schedule=None):
+    def fit(self, X, Y, epochs=50, batch=256, lr=1e-3, seed=2, schedule=None,
+            node_weight=1.0):
         """Adam on the mean squared error of the normalised targets: `epochs`
         at `lr`, or the stages of `schedule` ((epochs, lr), ...) one after
-        another with the optimiser state carried over.  Returns the loss of
-        every epoch."""
-        self.mlp.set_normalisation(*normalisation(X, Y))
+        another with the optimiser state carried over; the wall node samples
+        weighted by node_weight^2 (normalisation).  Returns the loss of every
+        epoch."""
+        self.mlp.set_normalisation(*normalisation(X, Y, node_weight))
         stages = schedule if schedule else ((epochs, lr),)
         hist = [self.mlp.train(X, Y, epochs=int(e), batch=batch, lr=float(r), seed=seed + k)
                 for k, (e, r) in enumerate(stages)]
