@@ -520,26 +520,18 @@ extern "C" int scft_batch_eval(scft_batch* b, const double* w, double* phi, doub
   return batch_eval_impl(b, w, phi, r, rnorm, status);
 }
 
+static int time_sweeps(scft_batch* b, int reps, float* ms);
+
 extern "C" int scft_batch_time_eval(scft_batch* b, const double* w, int reps,
                                     double* ms_per_eval, double* march_ms) {
   if (!b || !w || reps < 1) return SCFT_E_ARG;
   HIPCHK(hipMemcpyAsync(b->d_w, w, sizeof(double) * b->total_u, hipMemcpyHostToDevice,
                         b->stream));
   HIPCHK(hipMemsetAsync(b->d_status, 0, sizeof(unsigned) * b->n_inst, b->stream));
-  SCFT_TRY(enqueue_sweep(b));  // warm-up
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, b->stream));
-  for (int i = 0; i < reps; i++) SCFT_TRY(enqueue_sweep(b));
-  HIPCHK(hipEventRecord(e1, b->stream));
-  HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (ms_per_eval) *ms_per_eval = ms / reps;
-  if (march_ms) *march_ms = ms / reps;
+  SCFT_TRY(time_sweeps(b, reps, &ms));
+  if (ms_per_eval) *ms_per_eval = ms;
+  if (march_ms) *march_ms = ms;
   return SCFT_OK;
 }
 
@@ -586,21 +578,36 @@ extern "C" int scft_batch_time_buckets(scft_batch* b, const double* w, int reps,
   return SCFT_OK;
 }
 
-// Wall time of `reps` whole sweeps of the current device field (HIP events
-// on the main stream, after one untimed sweep).
+// Wall time of `reps` whole sweeps of the current device field, launched as
+// the solve launches them (one sweep captured into a graph and replayed;
+// HIP events on the main stream, after one untimed replay).
 static int time_sweeps(scft_batch* b, int reps, float* ms) {
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_sweep(b);
-  if (rc == SCFT_OK && hipEventRecord(e0, b->stream) != hipSuccess) rc = SCFT_E_HIP;
-  for (int i = 0; i < reps && rc == SCFT_OK; i++) rc = enqueue_sweep(b);
+  hipError_t ce = hipStreamEndCapture(b->stream, &graph);
+  if (rc == SCFT_OK && ce != hipSuccess) rc = SCFT_E_HIP;
+  if (rc == SCFT_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+    rc = SCFT_E_HIP;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (rc == SCFT_OK &&
+      (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = SCFT_E_HIP;
+  if (rc == SCFT_OK && (hipGraphLaunch(exec, b->stream) != hipSuccess ||
+                        hipEventRecord(e0, b->stream) != hipSuccess))
+    rc = SCFT_E_HIP;
+  for (int i = 0; i < reps && rc == SCFT_OK; i++)
+    if (hipGraphLaunch(exec, b->stream) != hipSuccess) rc = SCFT_E_HIP;
   if (rc == SCFT_OK && (hipEventRecord(e1, b->stream) != hipSuccess ||
                         hipEventSynchronize(e1) != hipSuccess ||
                         hipEventElapsedTime(ms, e0, e1) != hipSuccess))
     rc = SCFT_E_HIP;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  (void)hipStreamSynchronize(b->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
   if (rc == SCFT_OK) *ms /= static_cast<float>(reps);
   return rc;
 }
