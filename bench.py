@@ -76,12 +76,12 @@ def log(msg):
 # 34 / 58).
 WORKLOADS = {
     "config5": dict(
-        n_inst=16384, seed=20261205, precond=0.6, node_weight=10.0,
+        n_inst=16384, seed=20261205, precond=0.6, node_weight=10.0, cuts=(512,),
         text="config5: {n} DN instances on graded meshes h(x) = h0 (1 + x/ell), h0 "
              "log-uniform [1e-4,1e-2], ell in [0.05,0.5], 64-4096 nodes, N in "
              "{{512,...,8192}}, L in [2,8], tol 1e-6"),
     "config3": dict(
-        n_inst=4096, seed=20261202, precond=0.5, node_weight=1.0,
+        n_inst=4096, seed=20261202, precond=0.5, node_weight=1.0, cuts=(256,),
         text="config3: {n} DN instances, m log-uniform [32,4096], N in {{512,1024,2048,4096}}, "
              "L in [2,8], tol 1e-6"),
 }
@@ -201,7 +201,7 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
     from w0."""
     from scft_amd import warmstart as WS
     if args.net_file and os.path.exists(args.net_file):
-        ws = WS.WarmStart(S, seed=11)
+        ws = WS.WarmStart(S, seed=11, cuts=WORKLOADS[args.workload]["cuts"])
         rec = ws.load(args.net_file)
         log(f"bench: warm start: network loaded from {args.net_file} ({rec})")
         return warm_solves(S, b, cfg, w0, args, ws, rec)
@@ -222,7 +222,7 @@ def warm_start_leg(S, WL, b, cfg, w0, args):
         part += 1
     X, Y = np.concatenate(Xs), np.concatenate(Ys)
     t_data = time.perf_counter() - t0
-    ws = WS.WarmStart(S, seed=11)
+    ws = WS.WarmStart(S, seed=11, cuts=WORKLOADS[args.workload]["cuts"])
     t0 = time.perf_counter()
     hist = ws.fit(X, Y, batch=256, seed=12, schedule=TRAIN_SCHEDULE,
                   node_weight=WORKLOADS[args.workload]["node_weight"])
@@ -282,7 +282,8 @@ def warm_solves(S, b, cfg, w0, args, ws, rec):
         "first_residual_q10_q50_q90": q(first_warm),
         "cold_first_residual_q10_q50_q90": q(first_cold),
         "network": f"{WS.DIMS[0]} -> 4 x 256 tanh -> {WS.DIMS[-1]}, FP32, device "
-                   "forward/backward/Adam kernels",
+                   "forward/backward/Adam kernels; one per mesh-size range, split at m = "
+                   + ", ".join(str(c) for c in WORKLOADS[args.workload]["cuts"]),
         "training_fields": rec.get("training_fields"), "epochs": rec.get("epochs"),
         "batch": 256, "lr_schedule": [list(st) for st in TRAIN_SCHEDULE],
         "node_weight": WORKLOADS[args.workload]["node_weight"],

@@ -429,6 +429,14 @@ typedef struct scft_warm_plan scft_warm_plan;
 int scft_warm_plan_create(scft_warm_plan **out, scft_batch *b, scft_mlp *net, int n_in,
                           const float *inputs, int n_grid, const double *grid_x, int n_node,
                           const int *node_idx, double node_scale, const int *wall_count);
+/* The same for a subset of the batch's instances (one network per range of
+ * mesh sizes, one plan each): row k of inputs, grid_x and wall_count belongs
+ * to instance sel[k] (0-based, increasing, 1 <= n_sel <= n_inst); the other
+ * instances' fields are left as they are. */
+int scft_warm_plan_create_sel(scft_warm_plan **out, scft_batch *b, scft_mlp *net, int n_sel,
+                              const int *sel, int n_in, const float *inputs, int n_grid,
+                              const double *grid_x, int n_node, const int *node_idx,
+                              double node_scale, const int *wall_count);
 int scft_warm_plan_predict(scft_warm_plan *p); /* synchronous on the batch's stream */
 int scft_warm_plan_destroy(scft_warm_plan *p);
 

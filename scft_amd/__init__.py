@@ -139,6 +139,9 @@ def _load():
     L.scft_mlp_forward.argtypes = [C.c_void_p, C.c_int, _fp, _fp]
     L.scft_warm_plan_create.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p, C.c_int,
                                         _fp, C.c_int, _dp, C.c_int, _ip, C.c_double, _ip]
+    L.scft_warm_plan_create_sel.argtypes = [C.POINTER(C.c_void_p), C.c_void_p, C.c_void_p,
+                                            C.c_int, _ip, C.c_int, _fp, C.c_int, _dp, C.c_int,
+                                            _ip, C.c_double, _ip]
     L.scft_warm_plan_predict.argtypes = [C.c_void_p]
     L.scft_warm_plan_destroy.argtypes = [C.c_void_p]
     L.scft_mlp_loss_grad.argtypes = [C.c_void_p, C.c_int, _fp, _fp, C.POINTER(C.c_double),
@@ -407,24 +410,35 @@ class WarmPlan:
     field (scft_warm_plan_*): the network's forward on inputs kept in HBM,
     then one kernel mapping each instance's outputs onto its unknowns.
 
-    inputs: n_inst x n_in raw features; grid_x: n_inst x n_grid points;
-    node_idx: the 1-based node numbers of the wall samples, node_scale their
-    asinh scale; wall_count: per instance, the leading unknowns taken from
-    the samples."""
+    inputs: one row of raw features per instance; grid_x: one row of
+    n_grid points per instance; node_idx: the 1-based node numbers of the
+    wall samples, node_scale their asinh scale; wall_count: per instance,
+    the leading unknowns taken from the samples.  sel (optional): the
+    batch's instances the rows belong to (increasing), when the plan covers
+    a subset; without it, every instance in order."""
 
-    def __init__(self, batch, mlp, inputs, grid_x, node_idx, node_scale, wall_count):
+    def __init__(self, batch, mlp, inputs, grid_x, node_idx, node_scale, wall_count, sel=None):
         inputs = _f32(inputs)
         grid_x = _f64(grid_x)
         node_idx = np.ascontiguousarray(node_idx, dtype=np.int32)
         wall_count = np.ascontiguousarray(wall_count, dtype=np.int32)
-        if inputs.ndim != 2 or inputs.shape[0] != len(batch.n) or grid_x.ndim != 2 or \
-                grid_x.shape[0] != len(batch.n) or len(wall_count) != len(batch.n):
+        rows = len(batch.n) if sel is None else len(sel)
+        if inputs.ndim != 2 or inputs.shape[0] != rows or grid_x.ndim != 2 or \
+                grid_x.shape[0] != rows or len(wall_count) != rows:
             raise ValueError("one row of inputs, grid points and wall count per instance")
         self._keep = (batch, mlp)
         h = C.c_void_p()
-        check(lib.scft_warm_plan_create(C.byref(h), batch.h, mlp.h, inputs.shape[1], inputs,
-                                        grid_x.shape[1], grid_x, len(node_idx), node_idx,
-                                        float(node_scale), wall_count), "scft_warm_plan_create")
+        if sel is None:
+            check(lib.scft_warm_plan_create(C.byref(h), batch.h, mlp.h, inputs.shape[1], inputs,
+                                            grid_x.shape[1], grid_x, len(node_idx), node_idx,
+                                            float(node_scale), wall_count),
+                  "scft_warm_plan_create")
+        else:
+            sel = np.ascontiguousarray(sel, dtype=np.int32)
+            check(lib.scft_warm_plan_create_sel(C.byref(h), batch.h, mlp.h, len(sel), sel,
+                                                inputs.shape[1], inputs, grid_x.shape[1], grid_x,
+                                                len(node_idx), node_idx, float(node_scale),
+                                                wall_count), "scft_warm_plan_create_sel")
         self.h = h
 
     def predict(self):

@@ -1010,11 +1010,14 @@ extern "C" int scft_batch_solve(scft_batch* b, const scft_anderson_opts* o, doub
 
 // ------------------------------------------- warm start into the batch
 // The network's outputs mapped onto the batch's unknowns on the device
-// (warmpredict.hip): the prediction writes the resident field directly.
+// (warmpredict.hip): the prediction writes the resident field directly.  A
+// plan covers every instance of the batch or a chosen subset (one network
+// per range of mesh sizes: one plan each).
 struct scft_warm_plan {
   scft_batch* b = nullptr;
   scft_mlp* net = nullptr;
-  int n_inst = 0, n_in = 0, n_out = 0, n_grid = 0, n_node = 0;
+  int n_inst = 0, n_in = 0, n_out = 0, n_grid = 0, n_node = 0;  // n_inst: rows
+  int* d_sel = nullptr;      // batch instance of each row (null: all, in order)
   double node_scale = 1.0;
   float* d_xn = nullptr;     // normalised inputs, n_inst x n_in
   double* d_grid = nullptr;  // n_inst x n_grid
@@ -1030,23 +1033,30 @@ extern "C" int scft_warm_plan_destroy(scft_warm_plan* p) {
   dfree(p->d_node_idx);
   dfree(p->d_wall);
   dfree(p->d_norm);
+  dfree(p->d_sel);
   delete p;
   return SCFT_OK;
 }
 
-extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_mlp* net,
-                                     int n_in, const float* inputs, int n_grid,
-                                     const double* grid_x, int n_node, const int* node_idx,
-                                     double node_scale, const int* wall_count) {
+extern "C" int scft_warm_plan_create_sel(scft_warm_plan** out, scft_batch* b, scft_mlp* net,
+                                         int n_sel, const int* sel, int n_in,
+                                         const float* inputs, int n_grid, const double* grid_x,
+                                         int n_node, const int* node_idx, double node_scale,
+                                         const int* wall_count) {
   if (!out || !b || !net || !inputs || !grid_x || !wall_count || n_in < 1) return SCFT_E_ARG;
   if (n_grid < 4 || n_grid > WARM_MAX_GRID || n_node < 0 || n_node > WARM_MAX_NODES ||
       (n_node > 0 && (!node_idx || n_node < 2)))
     return SCFT_E_ARG;
+  if (sel ? (n_sel < 1 || n_sel > b->n_inst) : n_sel != b->n_inst) return SCFT_E_ARG;
   *out = nullptr;
   for (int j = 0; j < n_node; j++)
     if (node_idx[j] < 1 || (j > 0 && node_idx[j] <= node_idx[j - 1])) return SCFT_E_ARG;
-  for (int k = 0; k < b->n_inst; k++) {
-    if (wall_count[k] < 0 || wall_count[k] > b->desc[k].n ||
+  for (int k = 0; k < n_sel; k++) {
+    // the instances of a subset: increasing, so each is written once
+    if (sel && (sel[k] < 0 || sel[k] >= b->n_inst || (k > 0 && sel[k] <= sel[k - 1])))
+      return SCFT_E_ARG;
+    const int inst = sel ? sel[k] : k;
+    if (wall_count[k] < 0 || wall_count[k] > b->desc[inst].n ||
         (wall_count[k] > 0 && (n_node < 2 || wall_count[k] > node_idx[n_node - 1])))
       return SCFT_E_ARG;
     const double* g = grid_x + static_cast<size_t>(k) * n_grid;
@@ -1056,7 +1066,7 @@ extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_m
   auto* p = new scft_warm_plan();
   p->b = b;
   p->net = net;
-  p->n_inst = b->n_inst;
+  p->n_inst = n_sel;
   p->n_in = n_in;
   p->n_grid = n_grid;
   p->n_node = n_node;
@@ -1071,7 +1081,8 @@ extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_m
   if (rc) return fail(rc);
   if ((rc = dalloc(&p->d_xn, xn.size())) || (rc = dalloc(&p->d_grid, (size_t)p->n_inst * n_grid)) ||
       (rc = dalloc(&p->d_node_idx, std::max(1, n_node))) || (rc = dalloc(&p->d_wall, p->n_inst)) ||
-      (rc = dalloc(&p->d_norm, 2 * static_cast<size_t>(p->n_out))))
+      (rc = dalloc(&p->d_norm, 2 * static_cast<size_t>(p->n_out))) ||
+      (sel && (rc = dalloc(&p->d_sel, p->n_inst))))
     return fail(rc);
   auto up = [&](void* dst, const void* src, size_t bytes) {
     return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream) == hipSuccess;
@@ -1080,10 +1091,20 @@ extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_m
       !up(p->d_grid, grid_x, sizeof(double) * p->n_inst * n_grid) ||
       (n_node > 0 && !up(p->d_node_idx, node_idx, sizeof(int) * n_node)) ||
       !up(p->d_wall, wall_count, sizeof(int) * p->n_inst) ||
+      (sel && !up(p->d_sel, sel, sizeof(int) * p->n_inst)) ||
       hipStreamSynchronize(b->stream) != hipSuccess)
     return fail(SCFT_E_HIP);
   *out = p;
   return SCFT_OK;
+}
+
+extern "C" int scft_warm_plan_create(scft_warm_plan** out, scft_batch* b, scft_mlp* net,
+                                     int n_in, const float* inputs, int n_grid,
+                                     const double* grid_x, int n_node, const int* node_idx,
+                                     double node_scale, const int* wall_count) {
+  if (!b) return SCFT_E_ARG;
+  return scft_warm_plan_create_sel(out, b, net, b->n_inst, nullptr, n_in, inputs, n_grid, grid_x,
+                                   n_node, node_idx, node_scale, wall_count);
 }
 
 extern "C" int scft_warm_plan_predict(scft_warm_plan* p) {
@@ -1110,6 +1131,7 @@ extern "C" int scft_warm_plan_predict(scft_warm_plan* p) {
   a.grid = p->d_grid;
   a.node_idx = p->d_node_idx;
   a.wall_count = p->d_wall;
+  a.sel = p->d_sel;
   a.node_scale = p->node_scale;
   a.n_inst = p->n_inst;
   a.n_out = n_out;

@@ -143,14 +143,15 @@ struct WarmMapArgs {
   const InstDesc* desc;
   const double* x;         // the batch's node coordinates
   double* w;               // out: the batch's field on unknowns
-  const float* y;          // network outputs, n_inst x n_out
+  const float* y;          // network outputs, one row per instance of the plan
   const float* ym;         // output shift and scale (nullable: none)
   const float* ys;
-  const double* grid;      // n_inst x n_grid grid points, increasing
+  const double* grid;      // n_grid grid points per row, increasing
   const int* node_idx;     // n_node sample node numbers, increasing
-  const int* wall_count;   // per instance
+  const int* wall_count;   // per row
+  const int* sel;          // batch instance of each row (nullable: row k is instance k)
   double node_scale;
-  int n_inst, n_out, n_grid, n_node;
+  int n_inst, n_out, n_grid, n_node;  // n_inst: rows
 };
 hipError_t launch_warm_map(const WarmMapArgs& a, hipStream_t s);
 

@@ -35,16 +35,17 @@ __device__ double spline_value(int n, const double* gx, const double* gy, const 
 }
 
 __global__ void __launch_bounds__(WM_THREADS) warm_map_kernel(WarmMapArgs a) {
-  const int inst = blockIdx.x;
+  const int row = blockIdx.x;  // row of the plan; the batch instance it writes:
+  const int inst = a.sel ? a.sel[row] : row;
   const int tid = threadIdx.x;
   __shared__ double gx[WARM_MAX_GRID], gy[WARM_MAX_GRID], M[WARM_MAX_GRID];
   __shared__ double lo[WARM_MAX_GRID], di[WARM_MAX_GRID], up[WARM_MAX_GRID],
       rhs[WARM_MAX_GRID];
   __shared__ double sv[WARM_MAX_NODES];
   const int ng = a.n_grid;
-  const float* yr = a.y + static_cast<long long>(inst) * a.n_out;
+  const float* yr = a.y + static_cast<long long>(row) * a.n_out;
   for (int i = tid; i < ng; i += WM_THREADS) {
-    gx[i] = a.grid[static_cast<long long>(inst) * ng + i];
+    gx[i] = a.grid[static_cast<long long>(row) * ng + i];
     const float v = a.ys ? yr[i] * a.ys[i] + a.ym[i] : yr[i];
     gy[i] = static_cast<double>(v);
   }
@@ -85,7 +86,7 @@ __global__ void __launch_bounds__(WM_THREADS) warm_map_kernel(WarmMapArgs a) {
   const InstDesc d = a.desc[inst];
   const double* xn = a.x + d.xo;
   double* w = a.w + d.uo;
-  const int nwall = a.wall_count[inst];
+  const int nwall = a.wall_count[row];
   for (int u = tid; u < d.n; u += WM_THREADS) {
     const int node = u + 1;
     double v;

@@ -82,25 +82,39 @@ def test_warm_start_on_held_out_instances_as_specified():
         assert np.abs(ea["phi"] - eb["phi"]).max() <= 3e-6
 
 
-def test_device_warm_plan_matches_host_mapping():
+@pytest.mark.parametrize("cuts", [(), (128, 1024)], ids=["one_network", "three_ranges"])
+def test_device_warm_plan_matches_host_mapping(cuts):
     """scft_warm_plan_predict (network forward on resident inputs, then the
     mapping kernel writing the batch's field) against the host path
     (network forward, host spline resampling, node samples): the same field
     to FP32 rounding of the network's outputs, on graded meshes whose first
-    unknowns come from the wall samples and on coarse ones without them."""
+    unknowns come from the wall samples and on coarse ones without them;
+    with one network per mesh-size range, one plan per range writes its
+    instances (scft_warm_plan_create_sel) and leaves the others alone."""
     import scft_amd as S
     from scft_amd import warmstart as WS, workloads as WL
     c5, c3 = WL.config5(n_inst=256, seed=31), WL.config3(n_inst=256, seed=32)
     cfg = {k: list(c5[k]) + list(c3[k]) for k in ("xs", "Ns", "pts", "w0")}
     prep = WS.Prepared(S, cfg)
-    ws = WS.WarmStart(S, seed=3)
+    ws = WS.WarmStart(S, seed=3, cuts=cuts)
     rng = np.random.default_rng(4)
-    Yfake = rng.standard_normal((64, WS.DIMS[-1])).astype(np.float32) * 3.0 + 1.0
-    ws.mlp.set_normalisation(*WS.normalisation(prep.F, Yfake))
+    for net in ws.nets:
+        Yfake = rng.standard_normal((64, WS.DIMS[-1])).astype(np.float32) * 3.0 + 1.0
+        net.set_normalisation(*WS.normalisation(prep.F, Yfake))
     host = ws.predict_unknowns(prep)
     b = S.Batch(cfg["xs"], cfg["Ns"], [p[1:] for p in cfg["pts"]], bc=S.BC_DN)
-    b.upload(np.zeros(b.total))
+    g = ws.group(prep.m)
+    assert len(np.unique(g)) == len(cuts) + 1
     plan = ws.device_plan(b, prep)
+    assert len(plan.plans) == len(cuts) + 1
+    for k, p in enumerate(plan.plans):  # each range's plan alone: its instances only
+        b.upload(np.full(b.total, 7.0))
+        p.predict()
+        part = b.download()["w"]
+        for i in range(len(cfg["xs"])):
+            if g[i] != k:
+                assert np.all(part[i] == 7.0), (k, i)
+    b.upload(np.zeros(b.total))
     plan.predict()
     dev = np.concatenate(b.download()["w"])
     plan.close()
@@ -138,5 +152,18 @@ def test_saved_network_predicts_the_same(tmp_path):
     np.savez(bad, **z)
     with pytest.raises(ValueError):
         b.load(bad)
+    # one network per mesh-size range: cuts and every network round trip
+    Xr = X.copy()
+    Xr[:, 2] = np.log(rng.choice([64, 200, 3000], size=len(X))).astype(np.float32)
+    c = WS.WarmStart(S, seed=3, cuts=(128, 1024))
+    c.fit(Xr, Y, epochs=2, batch=128, lr=1e-3, seed=4)
+    c.save(path, training_fields=512)
+    d = WS.WarmStart(S, seed=99, cuts=(128, 1024))
+    assert d.load(path) == {"training_fields": 512}
+    assert np.array_equal(c.predict_unknowns(prep), d.predict_unknowns(prep))
+    with pytest.raises(ValueError):
+        a.load(path)  # a one-network object refuses a file of three
+    for w in (a, b, c, d):
+        w.close()
     a.close()
     b.close()

@@ -62,7 +62,44 @@ def test_prepared_prediction_matches_the_per_instance_path(S):
             return Y
 
     ws = WS.WarmStart.__new__(WS.WarmStart)
-    ws.S, ws.mlp = S, Net()
+    ws.S, ws.cuts, ws.nets = S, (), [Net()]
     flat = ws.predict_unknowns(prep)
     ref = np.concatenate([w[1:] for w in WS.to_mesh(S, cfg["xs"], Y)])
     assert np.array_equal(flat, ref)
+
+
+def test_each_instance_takes_the_network_of_its_mesh_size_range(S):
+    """With cuts (128, 512) every instance is predicted by the network of its
+    element-count range ([0,128), [128,512), [512,...)), the same mapping as
+    one network applied to the rows so assembled."""
+    from scft_amd import warmstart as WS
+    cfg = WL.config3(n_inst=64, seed=5)
+    prep = WS.Prepared(S, cfg)
+    cuts = (128, 512)
+    rng = np.random.default_rng(6)
+    Ys = [rng.standard_normal((64, WS.DIMS[-1])).astype(np.float32) * 0.3 for _ in range(3)]
+
+    class Net:
+        def __init__(self, k):
+            self.k, self.rows = k, None
+
+        def forward(self, F):
+            self.rows = [int(np.flatnonzero((prep.F == f).all(axis=1))[0]) for f in F]
+            return Ys[self.k][self.rows]
+
+    ws = WS.WarmStart.__new__(WS.WarmStart)
+    ws.S, ws.cuts, ws.nets = S, cuts, [Net(k) for k in range(3)]
+    flat = ws.predict_unknowns(prep)
+    g = np.searchsorted(cuts, prep.m, side="right")
+    assert set(np.unique(g)) == {0, 1, 2}
+    assert np.array_equal(ws.group([127, 128, 511, 512]), [0, 1, 1, 2])
+    for k in range(3):
+        assert ws.nets[k].rows == list(np.flatnonzero(g == k))
+    Ymix = np.stack([Ys[g[i]][i] for i in range(64)])
+    ref = np.concatenate([w[1:] for w in WS.to_mesh(S, cfg["xs"], Ymix)])
+    assert np.array_equal(flat, ref)
+    assert np.array_equal(WS.elements_of(prep.F), prep.m)  # m from the log-m feature
+    with pytest.raises(ValueError):
+        WS.WarmStart(S, cuts=(128, 128))
+    with pytest.raises(ValueError):
+        WS.WarmStart(S, cuts=(0,))
