@@ -124,6 +124,11 @@ TRAIN_SCHEDULE = ((100, 1e-3), (100, 3e-4), (100, 1e-4), (100, 3e-5))
 # 100 -> 0.270, 0.249, 0.234, 0.242, 0.278 s (median first residual 7.9e-5
 # -> 5.1e-5 at 10).  On the uniform meshes of configuration 3, whose coarse
 # instances have no node samples, weight 1 stays best.
+# One network per mesh-size range (cuts in WORKLOADS, element counts): warm
+# solve of the production mix one network / split at 256 / 512 / 1024 ->
+# 238 / 233 / 228 / 233 ms; of configuration 3 one / split at 128 / 256 /
+# 512 -> 39-45 / 36-37 / 35-36 / 36 ms (median first residual of the
+# m >= 512 instances 1.3e-4 -> 3.7e-5 at 256).
 
 
 def solver_settings(workload):
