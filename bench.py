@@ -76,12 +76,12 @@ def log(msg):
 # 34 / 58).
 WORKLOADS = {
     "config5": dict(
-        n_inst=16384, seed=20261205, precond=0.6, node_weight=10.0, cuts=(512,),
+        n_inst=16384, seed=20261205, precond=0.6, node_weight=10.0, cuts=(512,), warm_inner=1,
         text="config5: {n} DN instances on graded meshes h(x) = h0 (1 + x/ell), h0 "
              "log-uniform [1e-4,1e-2], ell in [0.05,0.5], 64-4096 nodes, N in "
              "{{512,...,8192}}, L in [2,8], tol 1e-6"),
     "config3": dict(
-        n_inst=4096, seed=20261202, precond=0.5, node_weight=1.0, cuts=(256,),
+        n_inst=4096, seed=20261202, precond=0.5, node_weight=1.0, cuts=(256,), warm_inner=4,
         text="config3: {n} DN instances, m log-uniform [32,4096], N in {{512,1024,2048,4096}}, "
              "L in [2,8], tol 1e-6"),
 }
@@ -129,6 +129,14 @@ TRAIN_SCHEDULE = ((100, 1e-3), (100, 3e-4), (100, 1e-4), (100, 3e-5))
 # 238 / 233 / 228 / 233 ms; of configuration 3 one / split at 128 / 256 /
 # 512 -> 39-45 / 36-37 / 35-36 / 36 ms (median first residual of the
 # m >= 512 instances 1.3e-4 -> 3.7e-5 at 256).
+# Inner sweeps (warm_inner in WORKLOADS: evaluations per sweep of the
+# instances outside the bucket of most work; the same results, another
+# schedule): configuration 3's warm solve with 1 / 2 / 4 / 8 -> 35.0-38.8 /
+# 34.1-35.0 / 32.8-33.8 / 36.8-37.4 ms (9 -> 5 sweeps: the coarse meshes'
+# extra evaluations and fallbacks run beside the large instances' march);
+# the production mix's warm solve and both cold legs gain nothing (the
+# largest bucket's own evaluations set their sweep count) or lose to the
+# extra concurrent traffic (configuration 3 cold: 82 -> 87 ms at 4).
 
 
 def solver_settings(workload):
@@ -250,6 +258,7 @@ def warm_solves(S, b, cfg, w0, args, ws, rec):
     wopts.fallback_factor = 10.0
     for k, v in warm_overrides().items():  # from a predicted start
         setattr(wopts, k, v)
+    wopts.inner_sweeps = WORKLOADS[args.workload]["warm_inner"]
     w0_flat = b.join(w0)
     b.upload(w0_flat, w0_flat)  # the cold fields the fallback restarts from, resident in HBM
     plan = ws.device_plan(b, prep)  # network inputs and mapping tables, resident in HBM
@@ -292,6 +301,7 @@ def warm_solves(S, b, cfg, w0, args, ws, rec):
         "training_fields": rec.get("training_fields"), "epochs": rec.get("epochs"),
         "batch": 256, "lr_schedule": [list(st) for st in TRAIN_SCHEDULE],
         "node_weight": WORKLOADS[args.workload]["node_weight"],
+        "inner_sweeps": WORKLOADS[args.workload]["warm_inner"],
         "final_loss": rec.get("final_loss"), "train_seconds": rec.get("train_seconds"),
         "predict_ms_per_step": round(1e3 * tp / args.steps, 2),
         "fallbacks": int(np.sum((res["status"] & S.ST_FALLBACK) != 0)),

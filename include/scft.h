@@ -220,6 +220,16 @@ typedef struct scft_anderson_opts {
                              SCFT_ST_RATIONAL); from w0 with 0.5: ~25% fewer
                              evaluations than the linear form throughout.
                              0: the rational form from the first update */
+  int inner_sweeps;       /* k >= 1 (0 counts as 1): within each sweep after
+                             the first, the instances outside the size
+                             bucket of most work go through k evaluations
+                             and mixer updates, on their own streams while
+                             that bucket marches once.  Every instance takes
+                             the same evaluations and updates in the same
+                             order as with 1 (the results are the same
+                             bits); the solve needs fewer sweeps when small
+                             instances need more evaluations than large
+                             ones.  At most 16 (1) */
 } scft_anderson_opts;
 
 void scft_default_anderson(scft_anderson_opts *o);

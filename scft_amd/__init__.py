@@ -69,7 +69,7 @@ class AndersonOpts(C.Structure):
                 ("field_floor", C.c_double), ("gs_precond", C.c_double),
                 ("gs_field_weight", C.c_double), ("gs_switch", C.c_int),
                 ("gs_switch_growth", C.c_double), ("gs_shift", C.c_double),
-                ("gs_shift_below", C.c_double)]
+                ("gs_shift_below", C.c_double), ("inner_sweeps", C.c_int)]
 
 
 class BroydenStats(C.Structure):
@@ -225,12 +225,12 @@ def anderson_opts(history=10, beta=0.1, tol=1e-6, max_iter=5000, step_cap=0.5,
                   fallback_factor=0.0, check_every=8, progress_seconds=0.0, precond=0.0,
                   backtrack=0.0, max_halvings=30, field_floor=0.0, gs_precond=0.0,
                   gs_field_weight=0.25, gs_switch=0, gs_switch_growth=0.0, gs_shift=0.0,
-                  gs_shift_below=0.0):
+                  gs_shift_below=0.0, inner_sweeps=1):
     return AndersonOpts(history, beta, tol, max_iter, step_cap, restart_factor, regularization,
                         transform, floor, fallback_factor, check_every, progress_seconds,
                         precond, backtrack, max_halvings, field_floor, gs_precond,
                         gs_field_weight, gs_switch, gs_switch_growth, gs_shift,
-                        gs_shift_below)
+                        gs_shift_below, inner_sweeps)
 
 
 class Batch:

@@ -104,7 +104,8 @@ struct scft_batch {
   unsigned* d_status = nullptr;
   double* d_slabs = nullptr;
   long long slab_doubles = 0;
-  int* d_counters = nullptr;  // bucket queues, then the active count
+  int* d_counters = nullptr;  // bucket queues, the active count, then a slot the
+                              // inner mixer rounds count into (never read)
   // mixer
   int hist_S = 0;
   double *d_hw = nullptr, *d_hr = nullptr, *d_wcold = nullptr;
@@ -125,6 +126,8 @@ struct scft_batch {
   hipGraph_t sweep_graph = nullptr;
   hipGraphExec_t sweep_exec = nullptr;
   AndersonArgs sweep_args;
+  int sweep_inner = 1;
+  int top_bucket = -1;  // the non-empty bucket of most work (n N summed)
 };
 
 // Forget the cached sweep graph: the bucket launches or mixer buffers it
@@ -190,6 +193,7 @@ static int size_scratch(scft_batch* b) {
     order.push_back({work, static_cast<int>(i)});
   }
   std::sort(order.begin(), order.end(), std::greater<std::pair<double, int>>());
+  b->top_bucket = order.empty() ? -1 : order[0].second;
   // greedy: next bucket onto the least loaded stream
   double load[1 + scft_batch::kSide] = {};
   for (auto& o : order) {
@@ -282,6 +286,50 @@ static int enqueue_sweep(scft_batch* b) {
       HIPCHK(hipEventRecord(b->join[s - 1], b->side[s - 1]));
       HIPCHK(hipStreamWaitEvent(b->stream, b->join[s - 1], 0));
     }
+  return SCFT_OK;
+}
+
+// One sweep of a solve: the march of every bucket as in enqueue_sweep, the
+// mixer of every instance after the join.  With inner > 1 every bucket but
+// the one of most work (the sweep's critical path) goes on, on its own
+// stream, through inner - 1 further rounds of [mixer over its own instances,
+// march]: each instance still alternates one evaluation and one update, and
+// instances already done are skipped by both kernels, so the results are the
+// same bits as with inner = 1 and only the sweep in which an evaluation
+// happens changes.  The inner mixer rounds count active instances into a
+// slot of their own: the host's stop test reads the final mixer's count.
+static int enqueue_solve_sweep(scft_batch* b, const AndersonArgs& a, int inner) {
+  const int nb = static_cast<int>(b->buckets.size());
+  HIPCHK(launch_reset(b->d_counters, nb + 1, b->stream));
+  bool used[1 + scft_batch::kSide] = {true};
+  for (auto& bk : b->buckets)
+    if (!bk.items.empty()) used[bk.stream] = true;
+  HIPCHK(hipEventRecord(b->fork, b->stream));
+  for (int s = 1; s <= scft_batch::kSide; s++)
+    if (used[s]) HIPCHK(hipStreamWaitEvent(b->side[s - 1], b->fork, 0));
+  for (int i = 0; i < nb; i++) {
+    const Bucket& bk = b->buckets[i];
+    if (bk.items.empty()) continue;
+    const hipStream_t s = stream_of(b, bk.stream);
+    const MarchArgs ma = march_args(b, bk, i);
+    HIPCHK(launch_eval(b, bk.K, bk.W, ma, bk.grid, s));
+    if (i == b->top_bucket) continue;
+    AndersonArgs ai = a;
+    ai.order = bk.d_items;
+    ai.n_inst = static_cast<int>(bk.items.size());
+    ai.active = b->d_counters + nb + 1;
+    for (int r = 1; r < inner; r++) {
+      HIPCHK(launch_anderson(ai, s));
+      HIPCHK(launch_reset(b->d_counters + i, 1, s));  // the bucket's work queue
+      HIPCHK(launch_eval(b, bk.K, bk.W, ma, bk.grid, s));
+    }
+  }
+  for (int s = 1; s <= scft_batch::kSide; s++)
+    if (used[s]) {
+      HIPCHK(hipEventRecord(b->join[s - 1], b->side[s - 1]));
+      HIPCHK(hipStreamWaitEvent(b->stream, b->join[s - 1], 0));
+    }
+  HIPCHK(launch_anderson(a, b->stream));
   return SCFT_OK;
 }
 
@@ -402,7 +450,7 @@ extern "C" int scft_batch_create(scft_batch** out, const scft_batch_desc* dsc) {
       (rc = dalloc(&b->d_pw, b->total_pw)) || (rc = dalloc(&b->d_phi, b->total_u)) ||
       (rc = dalloc(&b->d_r, b->total_u)) || (rc = dalloc(&b->d_rnorm, b->n_inst)) ||
       (rc = dalloc(&b->d_status, b->n_inst)) ||
-      (rc = dalloc(&b->d_counters, b->buckets.size() + 1)))
+      (rc = dalloc(&b->d_counters, b->buckets.size() + 2)))
     return fail(rc);
   for (auto& bk : b->buckets) {
     if (bk.items.empty()) continue;
@@ -772,6 +820,7 @@ extern "C" void scft_default_anderson(scft_anderson_opts* o) {
   o->gs_switch_growth = 0.0;
   o->gs_shift = 0.0;
   o->gs_shift_below = 0.0;
+  o->inner_sweeps = 1;
 }
 
 static int ensure_mixer(scft_batch* b, int S) {
@@ -830,6 +879,8 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   if (!(o->gs_precond >= 0.0) || !(o->gs_field_weight >= 0.0) || o->gs_switch < 0 ||
       !(o->gs_switch_growth >= 0.0) || !(o->gs_shift >= 0.0) || !(o->gs_shift_below >= 0.0))
     return SCFT_E_ARG;
+  if (o->inner_sweeps < 0 || o->inner_sweeps > 16) return SCFT_E_ARG;
+  const int inner = std::max(1, o->inner_sweeps);
   SCFT_TRY(ensure_mixer(b, o->history + 1));
   if (o->gs_shift > 0.0 && o->gs_precond > 0.0 && !b->d_tri) {
     drop_sweep_graph(b);
@@ -911,13 +962,15 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
   HIPCHK(launch_anderson(a, b->stream));
   a.fallback_thresh = 0.0;  // only the first evaluation may be rejected
 
-  // remaining sweeps: [reset, march per bucket, mixer] captured once
-  if (b->sweep_exec && std::memcmp(&a, &b->sweep_args, sizeof a) != 0) drop_sweep_graph(b);
+  // remaining sweeps: [reset, march per bucket (inner rounds beside the
+  // largest bucket), mixer] captured once
+  if (b->sweep_exec &&
+      (std::memcmp(&a, &b->sweep_args, sizeof a) != 0 || b->sweep_inner != inner))
+    drop_sweep_graph(b);
   if (!b->sweep_exec) {
     hipGraph_t graph = nullptr;
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    int rc = enqueue_sweep(b);
-    if (rc == SCFT_OK && launch_anderson(a, b->stream) != hipSuccess) rc = SCFT_E_HIP;
+    int rc = enqueue_solve_sweep(b, a, inner);
     hipError_t ce = hipStreamEndCapture(b->stream, &graph);
     if (rc != SCFT_OK) {
       if (graph) (void)hipGraphDestroy(graph);
@@ -927,6 +980,7 @@ extern "C" int scft_batch_solve_resident(scft_batch* b, const scft_anderson_opts
     b->sweep_graph = graph;
     HIPCHK(hipGraphInstantiate(&b->sweep_exec, graph, nullptr, nullptr, 0));
     b->sweep_args = a;
+    b->sweep_inner = inner;
   }
   hipGraphExec_t exec = b->sweep_exec;
   if (!b->h_active) HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&b->h_active), sizeof(int)));

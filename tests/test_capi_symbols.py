@@ -52,3 +52,18 @@ def test_library_exports_every_declared_symbol():
     funcs, data = declared_symbols()
     missing = [n for n in sorted(funcs | data) if not hasattr(lib, n)]
     assert not missing, f"declared in scft.h but not exported: {missing}"
+
+
+def test_binding_options_match_the_library_defaults():
+    """scft_default_anderson (host only) fills the binding's AndersonOpts
+    field by field as scft_amd.anderson_opts() does: the ctypes layout of
+    scft_anderson_opts agrees with the header, its last field included."""
+    _lib()
+    import scft_amd as S
+    o = S.AndersonOpts()
+    o.inner_sweeps = -7  # overwritten by the library
+    S.lib.scft_default_anderson(C.byref(o))
+    ref = S.anderson_opts()
+    for name, _ in S.AndersonOpts._fields_:
+        assert getattr(o, name) == getattr(ref, name), name
+    assert o.inner_sweeps == 1 and o.gs_shift_below == 0.0 and o.history == 10

@@ -281,6 +281,59 @@ def test_batched_solve_mixed_sizes_all_converge():
     b.close()
 
 
+def test_inner_sweeps_change_the_schedule_not_the_results():
+    """inner_sweeps = 4: the buckets outside the one of most work take up to
+    four evaluations and updates per sweep.  Every instance goes through the
+    same evaluations and updates in the same order, so the fields,
+    evaluation counts and statuses are the same bits as with one per sweep
+    (from w0, and from perturbed converged fields with the cold-field
+    fallback), in no more sweeps; when the small instances need more
+    evaluations than the large ones, in fewer."""
+    S = _S()
+    cfg = WL.config3(n_inst=96, seed=13, Ns=(512, 1024))
+    xs, Ns, pts = cfg["xs"], cfg["Ns"], cfg["pts"]
+    w0 = [w[1:] for w in cfg["w0"]]
+    b = S.Batch(xs, Ns, [p[1:] for p in pts], bc=S.BC_DN)
+    assert len(b.tuning()) >= 4  # several buckets
+    kw = dict(history=16, beta=1.0, tol=1e-6, max_iter=300, step_cap=0.0, transform=S.MIX_LOG,
+              precond=0.5, backtrack=1e30, check_every=1, floor=1e-8, field_floor=0.5,
+              gs_precond=0.5, gs_field_weight=0.25, gs_shift=3.0, gs_shift_below=0.5)
+    runs = {}
+    for inner in (1, 4):
+        b.upload(w0)
+        st = b.solve_resident(S.anderson_opts(inner_sweeps=inner, **kw))
+        runs[inner] = (b.download(), st["sweeps"])
+    (a, sa), (c, sc) = runs[1], runs[4]
+    assert a["converged"] == len(xs)
+    assert np.array_equal(a["iters"], c["iters"]) and np.array_equal(a["status"], c["status"])
+    for k in range(len(xs)):
+        assert np.array_equal(a["w"][k], c["w"][k]), k
+    assert sc <= sa
+    # predicted-like starts: converged fields with a smooth 1e-4 bump, the
+    # larger bump on the smaller meshes (they need more evaluations), and
+    # the cold fields for the fallback
+    n = np.array([len(x) - 1 for x in xs])
+    eps = np.where(n < 256, 3e-3, 1e-5)
+    starts = [w * (1.0 + e * np.cos(np.pi * x[1:] / x[-1])) for w, x, e in zip(a["w"], xs, eps)]
+    wkw = dict(kw, history=8, gs_shift_below=0.0, fallback_factor=10.0)
+    runs = {}
+    for inner in (1, 4):
+        b.upload(starts, w0)
+        st = b.solve_resident(S.anderson_opts(inner_sweeps=inner, **wkw))
+        runs[inner] = (b.download(), st["sweeps"])
+    (a, sa), (c, sc) = runs[1], runs[4]
+    print(f"perturbed starts: {sa} sweeps with one evaluation per sweep, {sc} with four "
+          f"inner rounds; evaluations mean {a['iters'].mean():.2f}", flush=True)
+    assert a["converged"] == len(xs)
+    assert np.array_equal(a["iters"], c["iters"]) and np.array_equal(a["status"], c["status"])
+    for k in range(len(xs)):
+        assert np.array_equal(a["w"][k], c["w"][k]), k
+    assert sc < sa
+    with pytest.raises(S.ScftError):
+        b.solve_resident(S.anderson_opts(inner_sweeps=17, **kw))
+    b.close()
+
+
 def test_nan_field_is_flagged_not_propagated():
     """A non-finite field marks its own instance (status NaN) and leaves the
     others untouched (host-forced condition, no device fault)."""
