@@ -81,7 +81,7 @@ WORKLOADS = {
              "log-uniform [1e-4,1e-2], ell in [0.05,0.5], 64-4096 nodes, N in "
              "{{512,...,8192}}, L in [2,8], tol 1e-6"),
     "config3": dict(
-        n_inst=4096, seed=20261202, precond=0.5, node_weight=1.0, cuts=(256,), warm_inner=4,
+        n_inst=4096, seed=20261202, precond=0.5, node_weight=10.0, cuts=(256,), warm_inner=4,
         text="config3: {n} DN instances, m log-uniform [32,4096], N in {{512,1024,2048,4096}}, "
              "L in [2,8], tol 1e-6"),
 }
@@ -123,7 +123,10 @@ TRAIN_SCHEDULE = ((100, 1e-3), (100, 3e-4), (100, 1e-4), (100, 3e-5))
 # prediction is set mostly there.  Warm solve there: weight 1, 3, 10, 30,
 # 100 -> 0.270, 0.249, 0.234, 0.242, 0.278 s (median first residual 7.9e-5
 # -> 5.1e-5 at 10).  On the uniform meshes of configuration 3, whose coarse
-# instances have no node samples, weight 1 stays best.
+# instances have no node samples, weight 1 was best for one network; with
+# the networks split by mesh size (below) weight 1 / 3 / 10 -> 35.1-35.7 /
+# 33.6-34.1 / 33.9-34.1 ms (10 kept: the lowest first residuals, 2.92 mean
+# evaluations); on the production mix split, 3 / 10 / 30 -> 234 / 228 / 244.
 # One network per mesh-size range (cuts in WORKLOADS, element counts): warm
 # solve of the production mix one network / split at 256 / 512 / 1024 ->
 # 238 / 233 / 228 / 233 ms; of configuration 3 one / split at 128 / 256 /
