@@ -221,10 +221,11 @@ typedef struct scft_anderson_opts {
                              evaluations than the linear form throughout.
                              0: the rational form from the first update */
   int inner_sweeps;       /* k >= 1 (0 counts as 1): within each sweep after
-                             the first, the instances outside the size
-                             bucket of most work go through k evaluations
-                             and mixer updates, on their own streams while
-                             that bucket marches once.  Every instance takes
+                             the first, the instances of the size buckets
+                             with at most a third of the work of the
+                             largest go through k evaluations and mixer
+                             updates, on their own streams while the
+                             largest marches once.  Every instance takes
                              the same evaluations and updates in the same
                              order as with 1 (the results are the same
                              bits); the solve needs fewer sweeps when small

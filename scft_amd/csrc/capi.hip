@@ -68,6 +68,7 @@ struct Bucket {
   long long slab_stride = 0;  // doubles per workgroup
   long long slab_base = 0;    // this bucket's region in the scratch (doubles)
   int stream = 0;             // which of the batch's streams launches it
+  double work = 0.0;          // sum of n N over its instances
 };
 
 // rows per lane of a new bucket (implicit Euler keeps ~8 doubles per row in
@@ -190,6 +191,7 @@ static int size_scratch(scft_batch* b) {
     need += bk.slab_stride * bk.grid;
     double work = 0.0;
     for (int k : bk.items) work += static_cast<double>(b->desc[k].n) * b->desc[k].N;
+    bk.work = work;
     order.push_back({work, static_cast<int>(i)});
   }
   std::sort(order.begin(), order.end(), std::greater<std::pair<double, int>>());
@@ -290,10 +292,14 @@ static int enqueue_sweep(scft_batch* b) {
 }
 
 // One sweep of a solve: the march of every bucket as in enqueue_sweep, the
-// mixer of every instance after the join.  With inner > 1 every bucket but
-// the one of most work (the sweep's critical path) goes on, on its own
-// stream, through inner - 1 further rounds of [mixer over its own instances,
-// march]: each instance still alternates one evaluation and one update, and
+// mixer of every instance after the join.  With inner > 1 every bucket with
+// at most a third of the work of the largest (whose march is the sweep's
+// critical path) goes on, on its own stream, through inner - 1 further
+// rounds of [mixer over its own instances, march]; a bucket with more
+// would make the full sweeps longer than its rounds save (configuration 3's
+// 2048 bucket, half the largest's work: its four rounds took 14.7 ms of a
+// sweep whose largest march took 7.3).  Each instance still alternates one
+// evaluation and one update, and
 // instances already done are skipped by both kernels, so the results are the
 // same bits as with inner = 1 and only the sweep in which an evaluation
 // happens changes.  The inner mixer rounds count active instances into a
@@ -307,13 +313,20 @@ static int enqueue_solve_sweep(scft_batch* b, const AndersonArgs& a, int inner) 
   HIPCHK(hipEventRecord(b->fork, b->stream));
   for (int s = 1; s <= scft_batch::kSide; s++)
     if (used[s]) HIPCHK(hipStreamWaitEvent(b->side[s - 1], b->fork, 0));
-  for (int i = 0; i < nb; i++) {
+  for (int i = 0; i < nb; i++) {  // every bucket's first march, smallest first
     const Bucket& bk = b->buckets[i];
     if (bk.items.empty()) continue;
+    HIPCHK(launch_eval(b, bk.K, bk.W, march_args(b, bk, i), bk.grid, stream_of(b, bk.stream)));
+  }
+  // then the inner rounds, queued behind every first march (the largest
+  // bucket's included) on streams that share a hardware queue
+  for (int i = 0; i < nb; i++) {
+    const Bucket& bk = b->buckets[i];
+    if (bk.items.empty() || inner < 2 || i == b->top_bucket ||
+        3.0 * bk.work > b->buckets[b->top_bucket].work)
+      continue;
     const hipStream_t s = stream_of(b, bk.stream);
     const MarchArgs ma = march_args(b, bk, i);
-    HIPCHK(launch_eval(b, bk.K, bk.W, ma, bk.grid, s));
-    if (i == b->top_bucket) continue;
     AndersonArgs ai = a;
     ai.order = bk.d_items;
     ai.n_inst = static_cast<int>(bk.items.size());
