@@ -93,8 +93,9 @@ struct scft_batch {
   hipStream_t stream = nullptr;  // main stream: copies, resets, the mixer
   // buckets run concurrently on the main stream plus kSide side streams:
   // one stream per bucket for the seven buckets of meshes up to 4096 nodes
-  // (a shared stream serialises its buckets: 13.2-13.7 ms per config-3 sweep
-  // on four streams against 12.5 ms on one stream each)
+  // (a shared stream serialises its buckets: a sweep of 4096 uniform meshes,
+  // m 32-4096, took 13.2-13.7 ms on four streams against 12.5 ms on one
+  // stream each)
   static constexpr int kSide = 7;
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
@@ -264,7 +265,7 @@ static MarchArgs march_args(const scft_batch* b, const Bucket& bk, int bucket_id
 // which stream capture turns into parallel graph branches).  The buckets are
 // launched smallest first, so that their short, latency-bound grids become
 // resident beside the large buckets' persistent grids.  That order matters
-// when buckets share a stream (a config-3 sweep of 4096 instances on four
+// when buckets share a stream (a sweep of 4096 uniform meshes on four
 // streams: 13.2 ms against 14.0 ms largest first); with a stream per bucket
 // both orders measure 12.5 ms.
 static hipStream_t stream_of(scft_batch* b, int s) { return s == 0 ? b->stream : b->side[s - 1]; }
@@ -296,13 +297,12 @@ static int enqueue_sweep(scft_batch* b) {
 // at most a third of the work of the largest (whose march is the sweep's
 // critical path) goes on, on its own stream, through inner - 1 further
 // rounds of [mixer over its own instances, march]; a bucket with more
-// would make the full sweeps longer than its rounds save (configuration 3's
-// 2048 bucket, half the largest's work: its four rounds took 14.7 ms of a
-// sweep whose largest march took 7.3).  Each instance still alternates one
-// evaluation and one update, and
-// instances already done are skipped by both kernels, so the results are the
-// same bits as with inner = 1 and only the sweep in which an evaluation
-// happens changes.  The inner mixer rounds count active instances into a
+// would make the full sweeps longer than its rounds save (on 4096 uniform
+// meshes the 2048 bucket, half the largest's work: its four rounds ran to
+// 14.7 ms of a sweep whose largest march took 7.3).  Each instance still
+// alternates one evaluation and one update, and instances already done are
+// skipped by both kernels, so the results are the same bits as with
+// inner = 1 and only the sweep in which an evaluation happens changes.  The inner mixer rounds count active instances into a
 // slot of their own: the host's stop test reads the final mixer's count.
 static int enqueue_solve_sweep(scft_batch* b, const AndersonArgs& a, int inner) {
   const int nb = static_cast<int>(b->buckets.size());

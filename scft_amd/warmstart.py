@@ -189,9 +189,10 @@ class WarmStart:
     cuts: element counts splitting the instances into mesh-size ranges
     [0, c1), [c1, c2), ..., each with a network of its own (the same shape,
     trained on the fields of its range alone).  One network learning coarse
-    and fine meshes together fits the fine ones worse: on configuration 3
-    (m 32-4096) a separate network for m >= 256 lowers their median first
-    residual from 1.3e-4 to 3.7e-5."""
+    and fine meshes together fits the fine ones worse: on 4096 uniform
+    meshes with m log-uniform in 32-4096 a separate network for m >= 256
+    lowers the median first residual of the m >= 512 instances from 1.3e-4
+    to 3.7e-5."""
 
     def __init__(self, S, seed: int = 1, cuts=()):
         self.S = S
