@@ -194,11 +194,15 @@ class WarmStart:
     lowers the median first residual of the m >= 512 instances from 1.3e-4
     to 3.7e-5."""
 
-    def __init__(self, S, seed: int = 1, cuts=()):
+    def __init__(self, S, seed: int = 1, cuts=(), range_epochs=None):
         self.S = S
         self.cuts = tuple(sorted(int(c) for c in cuts))
         if any(c < 1 for c in self.cuts) or len(set(self.cuts)) != len(self.cuts):
             raise ValueError(f"cuts {cuts}: distinct positive element counts")
+        # epochs of each range's network as a multiple of the schedule's
+        self.range_epochs = tuple(int(e) for e in (range_epochs or (1,) * (len(self.cuts) + 1)))
+        if len(self.range_epochs) != len(self.cuts) + 1 or min(self.range_epochs) < 1:
+            raise ValueError(f"range_epochs {range_epochs}: one positive factor per range")
         self.nets = [S.Mlp(DIMS, seed=seed) for _ in range(len(self.cuts) + 1)]
 
     @property
@@ -216,22 +220,23 @@ class WarmStart:
         at `lr`, or the stages of `schedule` ((epochs, lr), ...) one after
         another with the optimiser state carried over; the wall node samples
         weighted by node_weight^2 (normalisation).  Each range's network
-        trains on the rows of its range.  Returns the loss of every epoch
-        (averaged over the ranges by their row counts)."""
+        trains on the rows of its range, every stage's epochs multiplied by
+        the range's factor (range_epochs).  Returns each range's loss of
+        every epoch."""
         g = self.group(elements_of(X))
         stages = schedule if schedule else ((epochs, lr),)
-        total, hist = 0, 0.0
+        hist = []
         for k, net in enumerate(self.nets):
             rows = np.flatnonzero(g == k)
             if len(rows) == 0:
                 raise ValueError(f"no training fields in mesh-size range {k} (cuts {self.cuts})")
             Xk, Yk = X[rows], Y[rows]
             net.set_normalisation(*normalisation(Xk, Yk, node_weight))
-            h = np.concatenate([net.train(Xk, Yk, epochs=int(e), batch=batch, lr=float(r),
-                                          seed=seed + j) for j, (e, r) in enumerate(stages)])
-            hist = hist + len(rows) * h
-            total += len(rows)
-        return hist / total
+            f = self.range_epochs[k]
+            hist.append(np.concatenate([net.train(Xk, Yk, epochs=f * int(e), batch=batch,
+                                                  lr=float(r), seed=seed + j)
+                                        for j, (e, r) in enumerate(stages)]))
+        return hist[0] if len(hist) == 1 else hist
 
     def _forward(self, F, m) -> np.ndarray:
         g = self.group(m)
