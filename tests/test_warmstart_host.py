@@ -103,3 +103,7 @@ def test_each_instance_takes_the_network_of_its_mesh_size_range(S):
         WS.WarmStart(S, cuts=(128, 128))
     with pytest.raises(ValueError):
         WS.WarmStart(S, cuts=(0,))
+    with pytest.raises(ValueError):  # one epoch factor per range
+        WS.WarmStart(S, cuts=(128,), range_epochs=(2,))
+    with pytest.raises(ValueError):
+        WS.WarmStart(S, cuts=(128,), range_epochs=(0, 1))
