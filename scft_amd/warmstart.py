@@ -221,8 +221,8 @@ class WarmStart:
         another with the optimiser state carried over; the wall node samples
         weighted by node_weight^2 (normalisation).  Each range's network
         trains on the rows of its range, every stage's epochs multiplied by
-        the range's factor (range_epochs).  Returns each range's loss of
-        every epoch."""
+        the range's factor (range_epochs).  Returns the loss of every epoch:
+        one array, or with cuts a list of one array per range."""
         g = self.group(elements_of(X))
         stages = schedule if schedule else ((epochs, lr),)
         hist = []
